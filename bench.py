@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X PLFFT 1D transform path.
+
+One step = one batched transform pass over data already resident in HBM.
+The headline workload is C2C f32 forward, N=4096, batch 32768 (2^27 complex
+elements, 1 GiB in + 1 GiB out), out-of-place, unnormalized; further cases of
+the same family are reported under "cases". Prints ONE JSON line.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "perf-libs-fft_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def flops_c2c(n: int, batch: int) -> float:
+    return 5.0 * n * math.log2(n) * batch
+
+
+def time_plan(torch, plan, x, y, steps: int, warmup: int, adjoint: bool = False):
+    """Mean ms per launch over `steps` launches, HIP events on the launch stream."""
+    stream = torch.cuda.current_stream()
+    run = (lambda: plan.adjoint_async(x, y, stream=stream)) if adjoint else \
+        (lambda: plan.execute_async(x, y, stream=stream))
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        run()
+    end.record(stream)
+    end.synchronize()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def load_traffic(workload: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summaries."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes")
+
+
+def cpu_baseline(n: int, seconds: float = 10.0):
+    """The oracle's FP32 port of the reference's CT plan, OpenMP over a
+    bounded batch sample of the headline workload, on this host's cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle
+    from plfft_amd import rng
+
+    threads = min(16, os.cpu_count() or 1)
+    sample = 2048
+    x = rng.complex_uniform(0, n * sample)
+    y = np.empty_like(x)
+    oracle.c2c_batched_f32(x, n, -1, sample, 1, n, 1, n, y, threads)  # warm-up
+    iters, t0 = 0, time.perf_counter()
+    while True:
+        oracle.c2c_batched_f32(x, n, -1, sample, 1, n, 1, n, y, threads)
+        iters += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    gflops = flops_c2c(n, sample) * iters / dt / 1e9
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"C2C f32 fwd N={n}, {sample} transforms x {iters} passes "
+                      f"({dt:.1f} s), oracle FP32 port of the reference CT plan, OpenMP"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 1 GiB device copy (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the batch "
+              "shards across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import plfft_amd as P
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+
+    total = 1 << 27  # complex elements per case
+    x = torch.empty(total, dtype=torch.complex64, device="cuda")
+    y = torch.empty(total, dtype=torch.complex64, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    xr = x.view(torch.float32)
+    xr.uniform_(-1.0, 1.0, generator=g)  # synthetic U[-1,1) real/imag parts
+    torch.cuda.synchronize()
+
+    if args.copy_probe:
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": total * 8, "copy_probe_bytes_written": total * 8}))
+        return 0
+
+    # ---- headline: C2C f32 forward N=4096, B=32768, out-of-place
+    n, batch = 4096, total // 4096
+    plan = P.Plan("c2c", n, P.PLFFT_FORWARD, "f32", howmany=batch)
+    ms = time_plan(torch, plan, x, y, args.steps, args.warmup)
+    value = flops_c2c(n, batch) / (ms * 1e-3) / 1e9
+    alg_bytes = 2 * n * 8 * batch
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    workload = f"c2c_f32_fwd_n{n}_b{batch}_oop"
+
+    cases = []
+    if not args.headline_only:
+        specs = [
+            ("c2c", 256, "f32", -1, False), ("c2c", 1024, "f32", -1, False),
+            ("c2c", 4096, "f32", 1, False), ("c2c", 4096, "f32", -1, True),
+            ("c2c", 16384, "f32", -1, False), ("c2c", 65536, "f32", -1, False),
+            ("c2c", 1 << 20, "f32", -1, False), ("c2c", 4096, "f64", -1, False),
+            ("r2c", 4096, "f32", -1, False), ("c2r", 4096, "f32", 1, False),
+        ]
+        for kind, cn, prec, cdir, inplace in specs:
+            if prec == "f64":
+                cb = total // 2 // cn
+            elif kind in ("r2c", "c2r"):
+                cb = total // cn
+            else:
+                cb = total // cn
+            if cn == 1024 and kind == "c2c" and prec == "f32":
+                cb = 4096  # the reference's CPU-runnable configuration
+            try:
+                cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace)
+            except P.PlfftError as e:
+                cases.append({"kind": kind, "n": cn, "dtype": prec, "batch": cb,
+                              "status": f"unsupported: {e}"})
+                continue
+            dst = x if inplace else y
+            cms = time_plan(torch, cplan, x, dst, max(10, args.steps // 2), args.warmup)
+            esz = 16 if prec == "f64" else 8
+            if kind == "c2c":
+                cbytes = 2 * cn * esz * cb
+                cfl = flops_c2c(cn, cb)
+            else:
+                cbytes = (cn * esz // 2 + (cn // 2 + 1) * esz) * cb
+                cfl = flops_c2c(cn, cb) / 2
+            gbs = cbytes / (cms * 1e-3) / 1e9
+            cases.append({"kind": kind, "n": cn, "dtype": prec, "batch": cb,
+                          "dir": "fwd" if cdir < 0 else "bwd",
+                          "placement": "in-place" if inplace else "out-of-place",
+                          "ms": round(cms, 4), "GFLOP/s": round(cfl / (cms * 1e-3) / 1e9, 1),
+                          "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                          "plan": cplan.describe()})
+            del cplan
+            if inplace:  # restore the synthetic input
+                xr.uniform_(-1.0, 1.0, generator=g)
+                torch.cuda.synchronize()
+
+    out = {
+        "metric": "batched C2C f32 GFLOP/s (5 N log2 N) and %HBM peak",
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (device U[-1,1) real and imaginary parts, torch generator seed 0)",
+        "config": {"workload": workload, "n": n, "batch": batch, "direction": "forward",
+                   "placement": "out-of-place", "layout": "contiguous, stride 1, distance N",
+                   "parallelism": "single GPU (batch-sharded design, no collectives)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(workload)},
+        "plan": plan.describe(),
+        "cases": cases,
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(n)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"value": None, "error": str(e)}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

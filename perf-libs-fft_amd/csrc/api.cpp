@@ -1,0 +1,530 @@
+// api.cpp -- the C ABI of include/plfft.h and include/plfft_gpu.h.
+//
+// Configuration objects, defaults, status strings and the validation order
+// follow the reference implementation (src/plfft.cpp:22-526); plan creation
+// hands the validated problem to the GPU planner, and execute launches the
+// plan's kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/plfft.h"
+#include "../../include/plfft_gpu.h"
+#include "plan.hpp"
+
+namespace {
+
+enum transform_kind_t { TRANSFORM_C2C = 0, TRANSFORM_R2C = 1, TRANSFORM_C2R = 2, TRANSFORM_R2R = 3 };
+
+}  // namespace
+
+// src/plfft.cpp:33-46
+struct plfft_config_t {
+  int64_t n;
+  int64_t howmany;
+  int64_t istride;
+  int64_t idist;
+  int64_t ostride;
+  int64_t odist;
+  int transform_kind;
+  int payload;  // direction (C2C) or r2r kind (R2R)
+  plfft_data_type_t data_type;
+  plfft_io_alias_t alias;
+  plfft_sme_mode_t sme_mode;
+  plfft_rigor_t rigor;
+};
+
+struct plfft_plan_t {
+  plf::Problem prob;
+  int device = 0;
+  std::unique_ptr<plf::Node> node;
+  // adjoint node, built on first use
+  mutable std::mutex adj_mu;
+  mutable std::unique_ptr<plf::Node> adj;
+  mutable bool adj_tried = false;
+  // workspace pool for concurrent synchronous executions
+  mutable std::mutex ws_mu;
+  mutable std::vector<void *> ws_free;
+  mutable void *ws_default = nullptr;
+  size_t ws_bytes = 0;
+  // element sizes (bytes) of the input and output sides
+  size_t in_elem = 0, out_elem = 0;
+  int64_t in_count = 0, out_count = 0;  // logical elements per transform
+
+  ~plfft_plan_t() {
+    for (void *p : ws_free) (void)hipFree(p);
+    if (ws_default) (void)hipFree(ws_default);
+  }
+};
+
+[[noreturn]] static void out_of_memory() {
+  std::fputs("Fatal error: out of memory\n", stderr);
+  std::abort();
+}
+
+// ---------------------------------------------------------------------------
+// validation helpers (src/plfft.cpp:66-141)
+// ---------------------------------------------------------------------------
+static bool valid_batch(const plfft_config_t *c) {
+  if (c->howmany == 1) return c->istride > 0 && c->ostride > 0;
+  return c->howmany > 0 && c->istride > 0 && c->idist > 0 && c->ostride > 0 && c->odist > 0;
+}
+static bool valid_direction(int d) { return d == PLFFT_FORWARD || d == PLFFT_BACKWARD; }
+static bool valid_r2r_kind(int k) { return k >= PLFFT_R2R_DCT_1 && k <= PLFFT_R2R_HC2R; }
+static bool valid_transform_kind(const plfft_config_t *c) {
+  switch (c->transform_kind) {
+    case TRANSFORM_C2C:
+      return valid_direction(c->payload);
+    case TRANSFORM_R2C:
+    case TRANSFORM_C2R:
+      return true;
+    case TRANSFORM_R2R:
+      return valid_r2r_kind(c->payload);
+  }
+  return false;
+}
+static bool valid_alias(const plfft_config_t *c) {
+  return c->alias == PLFFT_IO_NO_ALIAS || c->alias == PLFFT_IO_MAY_ALIAS;
+}
+static bool valid_sme_mode(const plfft_config_t *c) {
+  return c->sme_mode == PLFFT_SME_DISABLED || c->sme_mode == PLFFT_SME_ENABLED;
+}
+static bool want_sme(const plfft_config_t *c) { return c->sme_mode == PLFFT_SME_ENABLED; }
+static bool valid_rigor(const plfft_config_t *c) {
+  return c->rigor >= PLFFT_ESTIMATE && c->rigor <= PLFFT_EXHAUSTIVE;
+}
+
+extern "C" {
+
+const char *plfft_status_to_string(plfft_status_t status) {
+  switch (status) {  // src/plfft.cpp:263-296
+    case PLFFT_OK: return "success";
+    case PLFFT_NULL_ARGUMENT: return "null argument";
+    case PLFFT_INVALID_SIZE: return "invalid transform size";
+    case PLFFT_UNSUPPORTED_BATCH: return "Unsupported batch or stride configuration";
+    case PLFFT_INVALID_TRANSFORM_KIND: return "invalid transform kind";
+    case PLFFT_INVALID_DATA_TYPE: return "invalid data type";
+    case PLFFT_INVALID_IO_ALIAS: return "invalid I/O alias mode";
+    case PLFFT_PLAN_CREATION_FAILED: return "plan creation failed";
+    case PLFFT_INVALID_SME_MODE: return "invalid SME mode";
+    case PLFFT_SME_NOT_ENABLED: return "SME support was not enabled when building PLFFT";
+    case PLFFT_SME_UNSUPPORTED_R2R: return "SME is not supported for real-to-real transforms";
+    case PLFFT_FP16_NOT_AVAILABLE: return "native FP16 support is required but unavailable on this CPU";
+    case PLFFT_SME_NOT_AVAILABLE: return "SME support is unavailable on this CPU";
+    case PLFFT_SME_UNSUPPORTED_FIXED_POINT: return "SME is not supported for fixed-point transforms";
+    case PLFFT_INVALID_PLANNER_RIGOR: return "invalid planning rigor level";
+  }
+  return "unknown status";
+}
+
+// src/plfft.cpp:303-319: every creator installs the same defaults, rigor
+// MEASURE included.
+static plfft_status_t create_config(plfft_config_t **config, int64_t n, int kind, int payload) {
+  if (config == nullptr) return PLFFT_NULL_ARGUMENT;
+  *config = nullptr;
+  void *mem = std::malloc(sizeof(plfft_config_t));
+  if (!mem) out_of_memory();
+  plfft_config_t *c = new (mem) plfft_config_t;
+  c->n = n;
+  c->howmany = 1;
+  c->istride = 1;
+  c->idist = n;
+  c->ostride = 1;
+  c->odist = n;
+  c->transform_kind = kind;
+  c->payload = payload;
+  c->data_type = PLFFT_DATA_TYPE_FP32;
+  c->alias = PLFFT_IO_NO_ALIAS;
+  c->sme_mode = PLFFT_SME_DISABLED;
+  c->rigor = PLFFT_MEASURE;
+  *config = c;
+  return PLFFT_OK;
+}
+
+plfft_status_t plfft_config_create(plfft_config_t **config, int64_t n, plfft_direction_t direction) {
+  if (config == nullptr) return PLFFT_NULL_ARGUMENT;
+  *config = nullptr;
+  if (!valid_direction((int)direction)) return PLFFT_INVALID_TRANSFORM_KIND;
+  return create_config(config, n, TRANSFORM_C2C, (int)direction);
+}
+
+plfft_status_t plfft_config_create_r2c(plfft_config_t **config, int64_t n) {
+  return create_config(config, n, TRANSFORM_R2C, 0);
+}
+
+plfft_status_t plfft_config_create_c2r(plfft_config_t **config, int64_t n) {
+  return create_config(config, n, TRANSFORM_C2R, 0);
+}
+
+plfft_status_t plfft_config_create_r2r(plfft_config_t **config, int64_t n, plfft_r2r_kind_t r2r_kind) {
+  if (config == nullptr) return PLFFT_NULL_ARGUMENT;
+  *config = nullptr;
+  if (!valid_r2r_kind((int)r2r_kind)) return PLFFT_INVALID_TRANSFORM_KIND;
+  return create_config(config, n, TRANSFORM_R2R, (int)r2r_kind);
+}
+
+void plfft_config_destroy(plfft_config_t *config) {
+  if (!config) return;
+  config->~plfft_config_t();
+  std::free(config);
+}
+
+void plfft_config_set_data_type(plfft_config_t *c, plfft_data_type_t t) { c->data_type = t; }
+plfft_data_type_t plfft_config_get_data_type(const plfft_config_t *c) { return c->data_type; }
+void plfft_config_set_io_alias(plfft_config_t *c, plfft_io_alias_t a) { c->alias = a; }
+plfft_io_alias_t plfft_config_get_io_alias(const plfft_config_t *c) { return c->alias; }
+void plfft_config_set_sme_mode(plfft_config_t *c, plfft_sme_mode_t m) { c->sme_mode = m; }
+plfft_sme_mode_t plfft_config_get_sme_mode(const plfft_config_t *c) { return c->sme_mode; }
+void plfft_config_set_rigor(plfft_config_t *c, plfft_rigor_t r) { c->rigor = r; }
+plfft_rigor_t plfft_config_get_rigor(const plfft_config_t *c) { return c->rigor; }
+
+void plfft_config_set_batch(plfft_config_t *c, int64_t howmany, int64_t istride, int64_t idist,
+                            int64_t ostride, int64_t odist) {
+  c->howmany = howmany;
+  c->istride = istride;
+  c->idist = idist;
+  c->ostride = ostride;
+  c->odist = odist;
+}
+int64_t plfft_config_get_howmany(const plfft_config_t *c) { return c->howmany; }
+int64_t plfft_config_get_istride(const plfft_config_t *c) { return c->istride; }
+int64_t plfft_config_get_idist(const plfft_config_t *c) { return c->idist; }
+int64_t plfft_config_get_ostride(const plfft_config_t *c) { return c->ostride; }
+int64_t plfft_config_get_odist(const plfft_config_t *c) { return c->odist; }
+
+}  // extern "C"
+
+// Logical element counts and byte sizes of the input and output sides.
+static void side_sizes(const plf::Problem &p, size_t *in_elem, size_t *out_elem,
+                       int64_t *in_count, int64_t *out_count) {
+  size_t real_bytes = p.fixed_bits == 15 ? 2 : p.fixed_bits == 7 ? 1 : (size_t)plf::prec_real_bytes(p.prec);
+  size_t cplx_bytes = 2 * real_bytes;
+  switch (p.kind) {
+    case plf::K_C2C:
+      *in_elem = *out_elem = cplx_bytes;
+      *in_count = *out_count = p.n;
+      break;
+    case plf::K_R2C:
+      *in_elem = real_bytes;
+      *out_elem = cplx_bytes;
+      *in_count = p.n;
+      *out_count = p.n / 2 + 1;
+      break;
+    case plf::K_C2R:
+      *in_elem = cplx_bytes;
+      *out_elem = real_bytes;
+      *in_count = p.n / 2 + 1;
+      *out_count = p.n;
+      break;
+    default:
+      *in_elem = *out_elem = real_bytes;
+      *in_count = *out_count = p.n;
+      break;
+  }
+}
+
+extern "C" plfft_status_t plfft_plan_create_from_config(plfft_plan_t **plan, const plfft_config_t *config) {
+  // validation order of src/plfft.cpp:427-517
+  if (!plan) return PLFFT_NULL_ARGUMENT;
+  *plan = nullptr;
+  if (!config) return PLFFT_NULL_ARGUMENT;
+  if (!(config->n > 0)) return PLFFT_INVALID_SIZE;
+  if (!valid_batch(config)) return PLFFT_UNSUPPORTED_BATCH;
+  if (!valid_transform_kind(config)) return PLFFT_INVALID_TRANSFORM_KIND;
+  if (!valid_alias(config)) return PLFFT_INVALID_IO_ALIAS;
+  if (!valid_sme_mode(config)) return PLFFT_INVALID_SME_MODE;
+  if (want_sme(config) && config->transform_kind == TRANSFORM_R2R) return PLFFT_SME_UNSUPPORTED_R2R;
+  if (want_sme(config) &&
+      (config->data_type == PLFFT_DATA_TYPE_Q7 || config->data_type == PLFFT_DATA_TYPE_Q15))
+    return PLFFT_SME_UNSUPPORTED_FIXED_POINT;
+  // This build has no SME support (a reference build without PLFFT_ENABLE_SME).
+  if (want_sme(config)) return PLFFT_SME_NOT_ENABLED;
+  // FP16 is native on the GPU: PLFFT_FP16_NOT_AVAILABLE never applies.
+  if (!valid_rigor(config)) return PLFFT_INVALID_PLANNER_RIGOR;
+
+  plf::Problem p;
+  switch ((int)config->data_type) {
+    case PLFFT_DATA_TYPE_FP32: p.prec = plf::P_F32; break;
+    case PLFFT_DATA_TYPE_FP64: p.prec = plf::P_F64; break;
+    case PLFFT_DATA_TYPE_FP16: p.prec = plf::P_F16; break;
+    case 5 /* PLFFT_DATA_TYPE_BF16 */: p.prec = plf::P_BF16; break;
+    case PLFFT_DATA_TYPE_Q7: p.prec = plf::P_F32; p.fixed_bits = 7; break;
+    case PLFFT_DATA_TYPE_Q15: p.prec = plf::P_F32; p.fixed_bits = 15; break;
+    default: return PLFFT_INVALID_DATA_TYPE;
+  }
+  // fixed-point R2R has no implementation (src/plfft.cpp:248-260)
+  if (p.fixed_bits && config->transform_kind == TRANSFORM_R2R) {
+    plf::set_last_error("fixed-point real-to-real transforms are not supported");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  p.kind = config->transform_kind;
+  if (p.kind == TRANSFORM_C2C) p.dir = config->payload;
+  else if (p.kind == TRANSFORM_R2C) p.dir = PLFFT_FORWARD;
+  else if (p.kind == TRANSFORM_C2R) p.dir = PLFFT_BACKWARD;
+  else p.r2r_kind = config->payload;
+  p.n = config->n;
+  p.howmany = config->howmany;
+  p.istride = config->istride;
+  p.ostride = config->ostride;
+  // howmany == 1 ignores the distances (src/plfft.cpp:75-81)
+  p.idist = config->howmany == 1 ? 0 : config->idist;
+  p.odist = config->howmany == 1 ? 0 : config->odist;
+  p.may_alias = config->alias == PLFFT_IO_MAY_ALIAS;
+  p.rigor = config->rigor;
+
+  int dev = 0, ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    plf::set_last_error("no HIP device available");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  if (hipGetDevice(&dev) != hipSuccess) {
+    plf::set_last_error("hipGetDevice failed");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  std::string why;
+  std::unique_ptr<plf::Node> node = plf::build_node(p, &why);
+  if (!node) {
+    plf::set_last_error(why);
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  void *mem = std::malloc(sizeof(plfft_plan_t));
+  if (!mem) out_of_memory();
+  plfft_plan_t *pl = new (mem) plfft_plan_t;
+  pl->prob = p;
+  pl->device = dev;
+  pl->ws_bytes = node->workspace_bytes();
+  pl->node = std::move(node);
+  side_sizes(p, &pl->in_elem, &pl->out_elem, &pl->in_count, &pl->out_count);
+  plf::set_last_error("");
+  *plan = pl;
+  return PLFFT_OK;
+}
+
+extern "C" void plfft_plan_destroy(plfft_plan_t *plan) {
+  if (!plan) return;
+  plan->~plfft_plan_t();
+  std::free(plan);
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    int cur = 0;
+    if (hipGetDevice(&cur) == hipSuccess && cur != dev) {
+      prev = cur;
+      (void)hipSetDevice(dev);
+    }
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+bool is_device_pointer(const void *p) {
+  hipPointerAttribute_t a;
+  std::memset(&a, 0, sizeof(a));
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+void *acquire_ws(const plfft_plan_t *pl) {
+  if (pl->ws_bytes == 0) return nullptr;
+  {
+    std::lock_guard<std::mutex> g(pl->ws_mu);
+    if (!pl->ws_free.empty()) {
+      void *p = pl->ws_free.back();
+      pl->ws_free.pop_back();
+      return p;
+    }
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, pl->ws_bytes) != hipSuccess) out_of_memory();
+  return p;
+}
+
+void release_ws(const plfft_plan_t *pl, void *p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> g(pl->ws_mu);
+  pl->ws_free.push_back(p);
+}
+
+[[noreturn]] void fatal_hip(const char *what, hipError_t e) {
+  std::fprintf(stderr, "plfft: %s failed: %s\n", what, hipGetErrorString(e));
+  std::abort();
+}
+
+// Span in bytes of a strided batched side.
+size_t span_bytes(int64_t count, int64_t stride, int64_t howmany, int64_t dist, size_t elem) {
+  int64_t span = (howmany - 1) * dist + (count - 1) * stride + 1;
+  return (size_t)span * elem;
+}
+
+void run_sync(const plfft_plan_t *pl, const plf::Node *node, const void *in, void *out,
+              size_t in_elem, int64_t in_count, int64_t is, int64_t id, size_t out_elem,
+              int64_t out_count, int64_t os, int64_t od) {
+  DeviceGuard guard(pl->device);
+  const int64_t hm = pl->prob.howmany;
+  const bool din = is_device_pointer(in), dout = is_device_pointer(out);
+  void *ws = acquire_ws(pl);
+  hipStream_t s = nullptr;  // legacy default stream: ordered after other blocking streams
+  hipError_t e;
+  if (din && dout) {
+    e = node->enqueue(in, out, ws, s);
+    if (e != hipSuccess) fatal_hip("kernel launch", e);
+  } else {
+    // Host buffers: stage through device memory, keeping any overlap.
+    const size_t ib = span_bytes(in_count, is, hm, id, in_elem);
+    const size_t ob = span_bytes(out_count, os, hm, od, out_elem);
+    const char *ilo = static_cast<const char *>(in), *ihi = ilo + ib;
+    char *olo = static_cast<char *>(out), *ohi = olo + ob;
+    const bool overlap = ilo < ohi && olo < ihi;
+    if (overlap) {
+      const char *lo = ilo < olo ? ilo : olo;
+      const char *hi = ihi > ohi ? ihi : ohi;
+      void *d = nullptr;
+      if ((e = hipMalloc(&d, (size_t)(hi - lo))) != hipSuccess) out_of_memory();
+      if ((e = hipMemcpy(d, lo, (size_t)(hi - lo), hipMemcpyHostToDevice)) != hipSuccess) fatal_hip("copy in", e);
+      char *db = static_cast<char *>(d);
+      e = node->enqueue(db + (ilo - lo), db + (olo - lo), ws, s);
+      if (e != hipSuccess) fatal_hip("kernel launch", e);
+      if ((e = hipMemcpy(olo, db + (olo - lo), ob, hipMemcpyDeviceToHost)) != hipSuccess) fatal_hip("copy out", e);
+      (void)hipFree(d);
+    } else {
+      void *di = nullptr, *dout_p = nullptr;
+      if (din) di = const_cast<void *>(in);
+      else {
+        if (hipMalloc(&di, ib) != hipSuccess) out_of_memory();
+        if ((e = hipMemcpy(di, in, ib, hipMemcpyHostToDevice)) != hipSuccess) fatal_hip("copy in", e);
+      }
+      if (dout) dout_p = out;
+      else {
+        if (hipMalloc(&dout_p, ob) != hipSuccess) out_of_memory();
+        // the gaps of a strided output must come back unchanged
+        if ((e = hipMemcpy(dout_p, out, ob, hipMemcpyHostToDevice)) != hipSuccess) fatal_hip("copy out-init", e);
+      }
+      e = node->enqueue(di, dout_p, ws, s);
+      if (e != hipSuccess) fatal_hip("kernel launch", e);
+      if (!dout) {
+        if ((e = hipMemcpy(out, dout_p, ob, hipMemcpyDeviceToHost)) != hipSuccess) fatal_hip("copy out", e);
+        (void)hipFree(dout_p);
+      }
+      if (!din) (void)hipFree(di);
+    }
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) fatal_hip("execute", e);
+  release_ws(pl, ws);
+}
+
+}  // namespace
+
+extern "C" void plfft_plan_execute(const plfft_plan_t *plan, const void *input, void *output) {
+  const plf::Problem &p = plan->prob;
+  run_sync(plan, plan->node.get(), input, output, plan->in_elem, plan->in_count, p.istride,
+           p.idist, plan->out_elem, plan->out_count, p.ostride, p.odist);
+}
+
+extern "C" size_t plfft_plan_get_workspace_size(const plfft_plan_t *plan) {
+  return plan ? plan->ws_bytes : 0;
+}
+
+static void *default_ws(const plfft_plan_t *pl) {
+  if (pl->ws_bytes == 0) return nullptr;
+  std::lock_guard<std::mutex> g(pl->ws_mu);
+  if (!pl->ws_default) {
+    if (hipMalloc(&pl->ws_default, pl->ws_bytes) != hipSuccess) out_of_memory();
+  }
+  return pl->ws_default;
+}
+
+extern "C" plfft_status_t plfft_plan_execute_async(const plfft_plan_t *plan, const void *input,
+                                                   void *output, void *workspace, void *stream) {
+  if (!plan || !input || !output) return PLFFT_NULL_ARGUMENT;
+  DeviceGuard guard(plan->device);
+  void *ws = workspace ? workspace : default_ws(plan);
+  hipError_t e = plan->node->enqueue(input, output, ws, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    plf::set_last_error(std::string("launch failed: ") + hipGetErrorString(e));
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  return PLFFT_OK;
+}
+
+// Adjoint problem: the linear map's transpose with input/output layouts
+// swapped.
+static const plf::Node *adjoint_node(const plfft_plan_t *pl) {
+  std::lock_guard<std::mutex> g(pl->adj_mu);
+  if (!pl->adj_tried) {
+    pl->adj_tried = true;
+    plf::Problem q = pl->prob;
+    std::swap(q.istride, q.ostride);
+    std::swap(q.idist, q.odist);
+    if (q.kind == plf::K_C2C) {
+      q.dir = -q.dir;
+      std::string why;
+      DeviceGuard guard(pl->device);
+      pl->adj = plf::build_node(q, &why);
+      if (!pl->adj) plf::set_last_error(why);
+    } else {
+      plf::set_last_error("adjoint not implemented for this transform kind yet");
+    }
+  }
+  return pl->adj.get();
+}
+
+extern "C" void plfft_plan_execute_adjoint(const plfft_plan_t *plan, const void *input, void *output) {
+  const plf::Node *n = adjoint_node(plan);
+  if (!n) {
+    std::fprintf(stderr, "plfft: adjoint unavailable: %s\n", plf::last_error());
+    std::abort();
+  }
+  const plf::Problem &p = plan->prob;
+  run_sync(plan, n, input, output, plan->out_elem, plan->out_count, p.ostride, p.odist,
+           plan->in_elem, plan->in_count, p.istride, p.idist);
+}
+
+extern "C" plfft_status_t plfft_plan_execute_adjoint_async(const plfft_plan_t *plan,
+                                                           const void *input, void *output,
+                                                           void *workspace, void *stream) {
+  if (!plan || !input || !output) return PLFFT_NULL_ARGUMENT;
+  const plf::Node *n = adjoint_node(plan);
+  if (!n) return PLFFT_PLAN_CREATION_FAILED;
+  DeviceGuard guard(plan->device);
+  void *ws = workspace ? workspace : default_ws(plan);
+  hipError_t e = n->enqueue(input, output, ws, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    plf::set_last_error(std::string("launch failed: ") + hipGetErrorString(e));
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  return PLFFT_OK;
+}
+
+extern "C" size_t plfft_plan_describe(const plfft_plan_t *plan, char *buf, size_t len) {
+  std::string s = plan ? plan->node->describe() : std::string("(null)");
+  if (buf && len) {
+    size_t k = s.size() < len - 1 ? s.size() : len - 1;
+    std::memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return s.size();
+}
+
+extern "C" const char *plfft_last_error(void) { return plf::last_error(); }

@@ -1,0 +1,80 @@
+// plan.hpp -- GPU plan tree for the 1D transform path.
+//
+// A plan is a tree of Nodes; each Node enqueues its kernels on a HIP stream.
+// Device resources (twiddle tables, chirps) are owned by the nodes and live
+// on the device that was current at plan creation. Workspace memory is passed
+// in by the caller so that one plan can run concurrently on several streams.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "registry.hpp"
+
+namespace plf {
+
+enum Kind : int { K_C2C = 0, K_R2C = 1, K_C2R = 2, K_R2R = 3 };
+
+// The validated problem, in the reference's terms (src/planner/problem.hpp:
+// 22-58): kind, precision, n, strides, batch, distances, direction.
+struct Problem {
+  int kind = K_C2C;
+  int dir = -1;       // -1 forward, +1 backward (C2C)
+  int r2r_kind = 0;   // plfft_r2r_kind_t for K_R2R
+  int prec = P_F32;   // storage precision (floating types)
+  int fixed_bits = 0; // 15 or 7 for Q15 / Q7 storage, 0 otherwise
+  int64_t n = 1, howmany = 1;
+  int64_t istride = 1, idist = 0, ostride = 1, odist = 0;
+  bool may_alias = false;
+  int rigor = 1;
+};
+
+// Owning device allocation.
+struct DevMem {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevMem() = default;
+  DevMem(const DevMem &) = delete;
+  DevMem &operator=(const DevMem &) = delete;
+  ~DevMem();
+  hipError_t alloc(size_t b);
+  hipError_t upload(const void *host, size_t b);
+};
+
+class Node {
+ public:
+  virtual ~Node() = default;
+  // Enqueue the transform of `in` into `out` on `s`. `work` holds at least
+  // workspace_bytes() of device memory (may be null when that is 0).
+  virtual hipError_t enqueue(const void *in, void *out, void *work,
+                             hipStream_t s) const = 0;
+  virtual size_t workspace_bytes() const { return 0; }
+  virtual std::string describe() const = 0;
+};
+
+// Thrown-free failure reporting for planning.
+struct PlanError {
+  std::string msg;
+};
+
+// Builds the node for a validated problem; returns null (and a reason) when
+// no GPU implementation exists.
+std::unique_ptr<Node> build_node(const Problem &p, std::string *why);
+
+// Twiddle table of a Stockham radix chain, FORWARD sign, FP64:
+// for stage s >= 1 with radix R and L = product of earlier radices,
+// tw[off_s + (m-1)*L + j] = exp(-2*pi*i*j*m/(L*R)), m in [1,R), j in [0,L).
+std::vector<double> stockham_twiddles(const std::vector<int> &chain);
+
+// exp(sign*2*pi*i*num/den) with num reduced exactly mod den, long double
+// evaluation rounded once to double.
+void unit_root(int64_t num, int64_t den, double *re, double *im);
+
+// Last error text for the calling thread (plan creation diagnostics).
+void set_last_error(const std::string &s);
+const char *last_error();
+
+}  // namespace plf

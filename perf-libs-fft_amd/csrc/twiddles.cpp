@@ -1,0 +1,68 @@
+// twiddles.cpp -- FP64 twiddle tables (host side).
+//
+// As in the reference (src/generate_twiddles.hpp:72-136), every factor is
+// computed in double from an integer-exact angle and rounded only when it is
+// stored in the kernel's compute precision.
+#include <cmath>
+#include <string>
+
+#include "plan.hpp"
+
+namespace plf {
+
+void unit_root(int64_t num, int64_t den, double *re, double *im) {
+  int64_t r = num % den;
+  if (r < 0) r += den;
+  // reduce to an angle in [0, pi/4] with exact integer arithmetic over 8*den
+  __int128 D = (__int128)den * 8;
+  __int128 t = (__int128)r * 8;
+  double ss = 1.0, cs = 1.0;
+  if (2 * t > D) {
+    t = D - t;
+    ss = -1.0;
+  }
+  if (4 * t > D) {
+    t = D / 2 - t;
+    cs = -1.0;
+  }
+  bool swap = false;
+  if (8 * t > D) {
+    t = D / 4 - t;
+    swap = true;
+  }
+  const long double a = 6.283185307179586476925286766559005768L *
+                        (long double)(int64_t)t / (long double)(int64_t)D;
+  long double c = cosl(a), s = sinl(a);
+  if (swap) {
+    long double tmp = c;
+    c = s;
+    s = tmp;
+  }
+  *re = (double)(cs * c);
+  *im = (double)(ss * s);
+}
+
+std::vector<double> stockham_twiddles(const std::vector<int> &chain) {
+  std::vector<double> tw;
+  int64_t L = chain.empty() ? 1 : chain[0];
+  for (size_t s = 1; s < chain.size(); ++s) {
+    const int64_t R = chain[s];
+    for (int64_t m = 1; m < R; ++m) {
+      for (int64_t j = 0; j < L; ++j) {
+        double re, im;
+        unit_root(j * m, L * R, &re, &im);
+        tw.push_back(re);
+        tw.push_back(-im);  // forward sign
+      }
+    }
+    L *= R;
+  }
+  return tw;
+}
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string &s) { g_last_error = s; }
+const char *last_error() { return g_last_error.c_str(); }
+
+}  // namespace plf

@@ -1,0 +1,160 @@
+"""C2C parity on the GPU: the HIP path through the C ABI vs the FP64 oracle.
+
+Cases follow the reference's tests: round trips returning n*x
+(test/advanced_test_utils.hpp:50-124), the c2c_f32 layout table
+(test/c2c_f32.cpp:10-27), known pairs (test/delta_constant_test.hpp,
+test/pure_tone_test.hpp) and MAY_ALIAS overlap (test/plfft_may_alias_test.cpp).
+"""
+import numpy as np
+import pytest
+
+import plfft_amd as P
+from plfft_amd import rng
+from gpu_util import CPLX_NP, TOL, default_tol, from_dev, relerr, span, to_dev
+
+pytestmark = pytest.mark.gpu
+
+POW2 = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+
+
+def _batch_for(n):
+    return max(1, min(64, (1 << 18) // n))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", POW2)
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_c2c_contiguous_vs_oracle(torch_cuda, orc, prec, n, direction):
+    torch = torch_cuda
+    if prec == "f64" and n > 8192:
+        pytest.skip("f64 single-kernel limit; covered by the large-N tests")
+    hm = _batch_for(n) + 3
+    x = rng.complex_normal(n, n * hm).astype(CPLX_NP[prec])
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    plan = P.Plan("c2c", n, direction, prec, howmany=hm)
+    plan.execute(xd, yd)
+    y = from_dev(torch, yd)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, direction, hm, 1, n, 1, n, n * hm)
+    assert relerr(y, ref) < TOL[prec], plan.describe()
+
+
+C2C_CASES = [
+    (2, 1, 1, 2, 1, 2), (8, 1, 1, 8, 1, 8), (16, 4, 1, 16, 1, 16), (8, 3, 2, 20, 2, 20),
+    (64, 2, 8, 600, 8, 600), (23, 49, 441, 9, 441, 9), (24, 4, 1, 32, 1, 32),
+    (512, 1, 1, 512, 1, 512), (512, 3, 1, 512, 1, 512), (512, 3, 3, 1, 3, 1),
+    (1024, 4, 1, 1024, 1, 1024), (1024, 4, 1, 1024, 1024, 1), (1058, 4, 1058, 1, 1058, 1),
+]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("case", C2C_CASES)
+def test_reference_layout_table_round_trip(torch_cuda, orc, prec, case):
+    """test/c2c_f32.cpp:10-27 replayed: fwd (NO_ALIAS) then bwd (MAY_ALIAS),
+    input (i+b+1)/n, result n*x within eps*4*n*log2(n); plus the forward
+    output against the FP64 oracle."""
+    torch = torch_cuda
+    n, hm, is_, id_, os_, od_ = case
+    isz, osz = span(n, is_, hm, id_), span(n, os_, hm, od_)
+    x = np.zeros(isz, np.complex128)
+    for b in range(hm):
+        x[b * id_ + is_ * np.arange(n)] = (np.arange(n) + b + 1) / n
+    x = x.astype(CPLX_NP[prec])
+    fwd = P.Plan("c2c", n, -1, prec, hm, is_, id_, os_, od_)
+    bwd = P.Plan("c2c", n, 1, prec, hm, os_, od_, is_, id_, may_alias=True)
+    xd = to_dev(torch, x)
+    yd = torch.zeros(osz, dtype=xd.dtype, device="cuda")
+    zd = torch.zeros(isz, dtype=xd.dtype, device="cuda")
+    fwd.execute(xd, yd)
+    bwd.execute(yd, zd)
+    y, z = from_dev(torch, yd), from_dev(torch, zd)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, is_, id_, os_, od_, osz)
+    assert relerr(y, ref) < TOL[prec]
+    tol = default_tol(prec, n)
+    for b in range(hm):
+        idx = b * id_ + is_ * np.arange(n)
+        assert np.max(np.abs(z[idx].real / n - x[idx].real)) <= tol
+
+
+@pytest.mark.parametrize("n", [16, 256, 4096])
+def test_known_pairs(torch_cuda, n):
+    torch = torch_cuda
+    plan = P.Plan("c2c", n, -1, "f32", howmany=3)
+    x = np.zeros((3, n), np.complex64)
+    x[0, 0] = 1                      # delta -> constant
+    x[1, :] = 1                      # constant -> n * delta
+    k0 = n // 3
+    x[2, k0] = 1                     # delta at k0 -> pure tone
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    plan.execute(xd, yd)
+    y = from_dev(torch, yd)
+    assert np.allclose(y[0], 1, atol=1e-6)
+    expect = np.zeros(n)
+    expect[0] = n
+    assert np.allclose(y[1], expect, atol=1e-6 * n)
+    tone = np.exp(-2j * np.pi * np.arange(n) * k0 / n)
+    assert np.max(np.abs(y[2] - tone)) < 1e-5
+
+
+@pytest.mark.parametrize("n", [64, 1024, 4096])
+def test_in_place_and_partial_overlap(torch_cuda, orc, n):
+    """MAY_ALIAS: exact overlap and +-1 element partial overlap of a
+    single member (test/plfft_may_alias_test.cpp:53-79)."""
+    torch = torch_cuda
+    x = rng.complex_uniform(0, n)
+    ref = orc.fft(x.astype(np.complex128))
+    plan = P.Plan("c2c", n, -1, "f32", may_alias=True)
+    for shift in (0, 1, -1):
+        buf = np.zeros(n + 2, np.complex64)
+        in_off = 1
+        buf[in_off:in_off + n] = x
+        d = to_dev(torch, buf)
+        base = d.data_ptr()
+        plan.execute(base + 8 * in_off, base + 8 * (in_off + shift))
+        out = from_dev(torch, d)[in_off + shift:in_off + shift + n]
+        assert relerr(out, ref) < TOL["f32"], shift
+
+
+def test_host_pointers_are_staged(orc):
+    """A reference caller passes host memory: the library stages it."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    n, hm = 1024, 5
+    x = rng.complex_uniform(4, n * hm)
+    y = np.zeros_like(x)
+    P.Plan("c2c", n, -1, "f32", howmany=hm).execute(x, y)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
+    assert relerr(y, ref) < TOL["f32"]
+
+
+def test_async_on_torch_stream(torch_cuda, orc):
+    torch = torch_cuda
+    n, hm = 4096, 40
+    x = rng.complex_uniform(9, n * hm)
+    plan = P.Plan("c2c", n, -1, "f32", howmany=hm)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        xd = torch.from_numpy(x).to("cuda", non_blocking=False)
+        yd = torch.empty_like(xd)
+        plan.execute_async(xd, yd, stream=s)
+    s.synchronize()
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
+    assert relerr(yd.cpu().numpy(), ref) < TOL["f32"]
+
+
+def test_adjoint_c2c(torch_cuda):
+    """<F x, y> == <x, F^H y> with F^H the adjoint execute."""
+    torch = torch_cuda
+    n, hm = 1024, 4
+    x = rng.complex_normal(1, n * hm).astype(np.complex128)
+    y = rng.complex_normal(2, n * hm).astype(np.complex128)
+    plan = P.Plan("c2c", n, -1, "f64", howmany=hm)
+    xd, yd = to_dev(torch, x), to_dev(torch, y)
+    fx, ay = torch.empty_like(xd), torch.empty_like(yd)
+    plan.execute(xd, fx)
+    plan.adjoint(yd, ay)
+    lhs = np.vdot(from_dev(torch, fx), y)
+    rhs = np.vdot(x, from_dev(torch, ay))
+    assert abs(lhs - rhs) < 1e-10 * abs(lhs)

@@ -91,6 +91,8 @@ def main() -> int:
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration)")
+    ap.add_argument("--only", default=None,
+                    help="run one case kind:n:dtype[:bwd][:inplace] (profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -115,10 +117,32 @@ def main() -> int:
     torch.cuda.synchronize()
 
     if args.copy_probe:
-        for _ in range(5):
+        for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": total * 8, "copy_probe_bytes_written": total * 8}))
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        reps = 20
+        start.record()
+        for _ in range(reps):
+            y.copy_(x)
+        end.record()
+        end.synchronize()
+        ms = start.elapsed_time(end) / reps
+        print(json.dumps({"copy_probe_bytes_read": total * 8, "copy_probe_bytes_written": total * 8,
+                          "ms": round(ms, 5), "GB/s": round(2 * total * 8 / (ms * 1e-3) / 1e9, 1)}))
+        return 0
+
+    if args.only:
+        parts = args.only.split(":")
+        kind, cn, prec = parts[0], int(parts[1]), parts[2]
+        cdir = 1 if "bwd" in parts[3:] else -1
+        inplace = "inplace" in parts[3:]
+        cb = total // cn if prec != "f64" else total // 2 // cn
+        cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace)
+        cms = time_plan(torch, cplan, x, x if inplace else y, args.steps, args.warmup)
+        print(json.dumps({"case": args.only, "batch": cb, "ms": round(cms, 5),
+                          "plan": cplan.describe()}))
         return 0
 
     # ---- headline: C2C f32 forward N=4096, B=32768, out-of-place

@@ -1,0 +1,231 @@
+// engine.hpp -- the LDS Stockham engine shared by every FFT kernel.
+//
+// An engine computes one length-N forward DFT cooperatively with TPT
+// threads (thread index t). Data enter through a load callback (stage 0
+// reads element e straight from wherever the caller says: global memory,
+// a chirp-multiplied input, a permuted gather, ...) and leave through a
+// store callback (the last stage writes output element e anywhere: global
+// memory, a twiddled workspace, ...), or through LDS when the caller needs a
+// cross-element epilogue (real-to-complex post-processing, transposed
+// stores). The stages in between exchange through LDS.
+//
+// Stage s of radix R with L = product of earlier radices is the
+// self-sorting (Stockham) DIT step
+//   v[m] = data[b + m*N/R] * W_{L*R}^{(b mod L)*m}
+//   V    = DFT_R(v)
+//   data'[(b div L)*L*R + (b mod L) + m*L] = V[m]
+// for butterfly b in [0, N/R). Stage 0 reads, and the last stage writes,
+// elements b + m*N/R: consecutive threads touch consecutive elements.
+//
+// All threads of the block must call run() (it contains barriers); threads
+// without a valid transform pass valid=false and their callbacks are not
+// invoked.
+#pragma once
+
+#include "fft_device.hpp"
+#include "kernel_args.hpp"
+
+namespace plf {
+
+// ---------------------------------------------------------------------------
+// compile-time radix chain
+// ---------------------------------------------------------------------------
+template <typename C, int TPT, int PAD, typename Ch>
+struct StaticEngine {
+  static constexpr int N = Ch::N;
+  static constexpr int NS = Ch::count;
+  static constexpr int LDS_PER = lds_padded<PAD>(N);
+  static constexpr bool USES_LDS = NS > 1;
+
+  template <int SI, bool TO_LDS, class Load, class Store>
+  static __device__ __forceinline__ void stage(cpx<C> *lds, const cpx<C> *__restrict__ tw,
+                                               int t, bool valid, Load &load, Store &store) {
+    constexpr int R = Ch::radix(SI);
+    constexpr int L = Ch::L(SI);
+    constexpr int NB = N / R;
+    constexpr int KB = (NB + TPT - 1) / TPT;
+    constexpr bool FIRST = (SI == 0);
+    constexpr bool LAST = (SI == NS - 1);
+    cpx<C> v[KB][R];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int b = t + k * TPT;
+      const bool act = (NB % TPT == 0) || (b < NB);
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        if constexpr (FIRST) {
+          v[k][m] = (act && valid) ? load(b + m * NB) : cpx<C>{C(0), C(0)};
+        } else {
+          v[k][m] = act ? lds[lds_pad<PAD>(b + m * NB)] : cpx<C>{C(0), C(0)};
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int b = t + k * TPT;
+      if constexpr (!FIRST) {
+        const int j = b % L;
+        const bool act = (NB % TPT == 0) || (b < NB);
+        if (act) {
+#pragma unroll
+          for (int m = 1; m < R; ++m)
+            v[k][m] = cmul(v[k][m], tw[Ch::tw_offset(SI) + (m - 1) * L + j]);
+        }
+      }
+      Dft<R, C>::run(v[k]);
+    }
+    if constexpr (!FIRST && (!LAST || TO_LDS)) __syncthreads();  // all reads done
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int b = t + k * TPT;
+      const bool act = (NB % TPT == 0) || (b < NB);
+      if constexpr (LAST && !TO_LDS) {
+        if (act && valid) {
+#pragma unroll
+          for (int m = 0; m < R; ++m) store(b + m * NB, v[k][m]);
+        }
+      } else if constexpr (LAST) {
+        if (act) {
+#pragma unroll
+          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(b + m * NB)] = v[k][m];
+        }
+      } else {
+        if (act) {
+          const int j = b % L;
+          const int base = (b / L) * L * R + j;
+#pragma unroll
+          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = v[k][m];
+        }
+      }
+    }
+    if constexpr (!LAST || TO_LDS) __syncthreads();
+    if constexpr (!LAST) stage<SI + 1, TO_LDS>(lds, tw, t, valid, load, store);
+  }
+
+  // Results stay in LDS (natural order, lds[lds_pad(e)]) when TO_LDS.
+  template <bool TO_LDS, class Load, class Store>
+  static __device__ __forceinline__ void run(cpx<C> *lds, const cpx<C> *tw, int t, bool valid,
+                                             Load &&load, Store &&store) {
+    stage<0, TO_LDS>(lds, tw, t, valid, load, store);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// runtime radix chain (descriptor in device memory). BIGP enables the
+// radix 17/19/23 butterflies, which need more registers than a 1024-thread
+// block can give each thread.
+// ---------------------------------------------------------------------------
+template <typename C, int TPT, int PTS, bool BIGP>
+struct DynamicEngine {
+  static constexpr int PAD = 5;
+
+  template <int R, bool TO_LDS, class Load, class Store>
+  static __device__ __forceinline__ void stage(const GenericDesc *__restrict__ d, int s,
+                                               cpx<C> *lds, const cpx<C> *__restrict__ tw,
+                                               int t, bool valid, Load &load, Store &store) {
+    constexpr int KMAX = (PTS + R - 1) / R;
+    const int n = d->n;
+    const int NB = n / R;
+    const int L = d->L[s];
+    const bool first = (s == 0);
+    const bool last = (s == d->ns - 1);
+    const cpx<C> *tws = tw + d->twoff[s];
+    cpx<C> v[KMAX][R];
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int b = t + k * TPT;
+      const bool act = b < NB;
+#pragma unroll
+      for (int m = 0; m < R; ++m) {
+        cpx<C> e = {C(0), C(0)};
+        if (act) {
+          if (first) {
+            if (valid) e = load(b + m * NB);
+          } else {
+            e = lds[lds_pad<PAD>(b + m * NB)];
+          }
+        }
+        v[k][m] = e;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int b = t + k * TPT;
+      if (!first && b < NB) {
+        const int j = b % L;
+#pragma unroll
+        for (int m = 1; m < R; ++m) v[k][m] = cmul(v[k][m], tws[(m - 1) * L + j]);
+      }
+      Dft<R, C>::run(v[k]);
+    }
+    if (!first && (!last || TO_LDS)) __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KMAX; ++k) {
+      const int b = t + k * TPT;
+      if (b < NB) {
+        if (last && !TO_LDS) {
+          if (valid) {
+#pragma unroll
+            for (int m = 0; m < R; ++m) store(b + m * NB, v[k][m]);
+          }
+        } else if (last) {
+#pragma unroll
+          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(b + m * NB)] = v[k][m];
+        } else {
+          const int j = b % L;
+          const int base = (b / L) * L * R + j;
+#pragma unroll
+          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = v[k][m];
+        }
+      }
+    }
+    if (!last || TO_LDS) __syncthreads();
+  }
+
+  template <bool TO_LDS, class Load, class Store>
+  static __device__ __forceinline__ void run(const GenericDesc *__restrict__ d, cpx<C> *lds,
+                                             const cpx<C> *tw, int t, bool valid, Load &&load,
+                                             Store &&store) {
+    const int ns = d->ns;
+    if (ns == 0) {  // n == 1: identity
+      if (TO_LDS) {
+        if (t == 0 && valid) lds[0] = load(0);
+        __syncthreads();
+      } else if (t == 0 && valid) {
+        store(0, load(0));
+      }
+      return;
+    }
+    for (int s = 0; s < ns; ++s) {
+      switch (d->radix[s]) {
+#define PLF_DYN_CASE(R)                                                   \
+  case R:                                                                 \
+    stage<R, TO_LDS>(d, s, lds, tw, t, valid, load, store);               \
+    break;
+        PLF_DYN_CASE(2)
+        PLF_DYN_CASE(3)
+        PLF_DYN_CASE(4)
+        PLF_DYN_CASE(5)
+        PLF_DYN_CASE(7)
+        PLF_DYN_CASE(8)
+        PLF_DYN_CASE(11)
+        PLF_DYN_CASE(13)
+        PLF_DYN_CASE(16)
+#undef PLF_DYN_CASE
+        case 17:
+          if constexpr (BIGP) stage<17, TO_LDS>(d, s, lds, tw, t, valid, load, store);
+          break;
+        case 19:
+          if constexpr (BIGP) stage<19, TO_LDS>(d, s, lds, tw, t, valid, load, store);
+          break;
+        case 23:
+          if constexpr (BIGP) stage<23, TO_LDS>(d, s, lds, tw, t, valid, load, store);
+          break;
+        default:
+          break;
+      }
+    }
+  }
+};
+
+}  // namespace plf

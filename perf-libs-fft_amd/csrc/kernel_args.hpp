@@ -1,19 +1,38 @@
-// kernel_args.hpp -- POD argument blocks passed by value to the kernels.
+// kernel_args.hpp -- POD argument block passed by value to every kernel.
 #pragma once
 
 #include <stdint.h>
 
 namespace plf {
 
-// Batched transform over `batch` members; element k of member b lives at
-// base[b*dist + k*stride] (complex elements for complex data).
-struct FusedArgs {
+// Runtime radix chain for the dynamic engine (stage s has radix[s],
+// L[s] = product of the earlier radices, its twiddles start at twoff[s]).
+constexpr int kMaxStages = 24;
+struct GenericDesc {
+  int n;
+  int ns;
+  int radix[kMaxStages];
+  int L[kMaxStages];
+  int twoff[kMaxStages];
+};
+
+// One argument block for all kernels; each kernel documents the fields it
+// reads. Strides and distances count logical elements.
+struct KArgs {
   const void *in;
   void *out;
-  const void *tw;  // per-stage twiddle table in the compute precision
-  int64_t batch;
+  const void *tw;            // Stockham twiddles (compute precision)
+  const GenericDesc *desc;   // dynamic-engine chain (device memory) or null
+  const void *aux0;          // kernel-specific tables (four-step twiddles, chirps, ...)
+  const void *aux1;
+  int64_t count;             // transforms (or columns / rows) in this launch
   int64_t istride, idist;
   int64_t ostride, odist;
+  int64_t n;                 // length handled by the engine
+  int64_t n1, n2;            // four-step split N = n1 * n2
+  int64_t bigdist;           // four-step: workspace distance between big transforms
+  int shift;                 // four-step twiddle split: W^e = hi[e >> shift] * lo[e & mask]
+  int flags;
 };
 
 }  // namespace plf

@@ -1,0 +1,171 @@
+// kernels.hpp -- batched transform kernels built on the Stockham engines.
+//
+// Engine policies:
+//   SEng<C, TPT, Rs...>  compile-time radix chain (fully specialised)
+//   DEng<C, TPT, PTS, TPB> runtime chain from a GenericDesc in device memory
+// Kernels (one argument block, KArgs):
+//   k_c2c  batched C2C, member = TPT-thread group      (replaces the
+//          reference's loop + cooley_tukey ab/ac plans,
+//          src/planner/strategies/{loop,cooley_tukey}.hpp)
+//   k_col  four-step pass A: length-n1 column DFTs over a tile of TPB
+//          consecutive columns, times W_N^{col*k1}, into the workspace
+//   k_row  four-step pass B: length-n2 row DFTs of the workspace with a
+//          transposed (column-contiguous) store through LDS
+// A backward transform swaps re/im at the global load and the final store.
+#pragma once
+
+#include "engine.hpp"
+
+namespace plf {
+
+template <typename C_, int TPT_, int... Rs>
+struct SEng {
+  using C = C_;
+  static constexpr int TPT = TPT_;
+  static constexpr int PAD = 5;
+  using Ch = Chain<Rs...>;
+  using E = StaticEngine<C, TPT, PAD, Ch>;
+  static constexpr int NFIX = Ch::N;
+  static __host__ __device__ constexpr int lds_per(int64_t) { return lds_padded<PAD>(Ch::N); }
+  template <bool TO_LDS, class Load, class Store>
+  static __device__ __forceinline__ void run(const GenericDesc *, cpx<C> *lds, const cpx<C> *tw,
+                                             int t, bool valid, Load &&load, Store &&store) {
+    E::template run<TO_LDS>(lds, tw, t, valid, load, store);
+  }
+};
+
+template <typename C_, int TPT_, int PTS_, int TPB_>
+struct DEng {
+  using C = C_;
+  static constexpr int TPT = TPT_;
+  static constexpr int PTS = PTS_;
+  static constexpr int PAD = 5;
+  static constexpr int NFIX = 0;
+  static constexpr bool BIGP = TPT_ <= 256 && TPB_ <= 16 && TPT_ * TPB_ <= 256;
+  using E = DynamicEngine<C, TPT, PTS, BIGP>;
+  static __host__ __device__ constexpr int lds_per(int64_t n) { return lds_padded<PAD>((int)n); }
+  template <bool TO_LDS, class Load, class Store>
+  static __device__ __forceinline__ void run(const GenericDesc *d, cpx<C> *lds, const cpx<C> *tw,
+                                             int t, bool valid, Load &&load, Store &&store) {
+    E::template run<TO_LDS>(d, lds, tw, t, valid, load, store);
+  }
+};
+
+template <typename C>
+__device__ __forceinline__ cpx<C> *dyn_lds() {
+  extern __shared__ unsigned char plf_dyn_lds[];
+  return reinterpret_cast<cpx<C> *>(plf_dyn_lds);
+}
+
+// ---------------------------------------------------------------------------
+// batched C2C: in[b*idist + e*istride] -> out[b*odist + k*ostride]
+// ---------------------------------------------------------------------------
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_c2c(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const E *in = static_cast<const E *>(a.in) + (valid ? tr * a.idist : 0);
+  E *out = static_cast<E *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  const int64_t is = a.istride, os = a.ostride;
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        cpx<C> v = ST::ld(in[(int64_t)e * is]);
+        return BWD ? cswap(v) : v;
+      },
+      [&](int e, cpx<C> v) { out[(int64_t)e * os] = ST::st(BWD ? cswap(v) : v); });
+}
+
+// W_N^e from the split table: hi[e >> shift] * lo[e & (2^shift - 1)].
+template <typename C>
+__device__ __forceinline__ cpx<C> split_twiddle(const cpx<C> *__restrict__ lo,
+                                                const cpx<C> *__restrict__ hi, int64_t e,
+                                                int shift) {
+  const int64_t mask = (int64_t(1) << shift) - 1;
+  return cmul(hi[e >> shift], lo[e & mask]);
+}
+
+// ---------------------------------------------------------------------------
+// four-step pass A. Column c2 of big transform bi (col = bi*n2 + c2, count =
+// chunk*n2 columns): x[bi*idist + (c2 + e*n2)*istride], e < n1. Output
+// k1 goes to ws[bi*bigdist + k1*n2 + c2] times W_N^{c2*k1}. Threads are
+// column-fastest so that a wave reads and writes TPB-wide contiguous runs.
+// ---------------------------------------------------------------------------
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_col(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  const int g = threadIdx.x % TPB;
+  const int t = threadIdx.x / TPB;
+  const int64_t col = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = col < a.count;
+  const int64_t bi = valid ? col / a.n2 : 0;
+  const int64_t c2 = valid ? col % a.n2 : 0;
+  const E *in = static_cast<const E *>(a.in) + bi * a.idist + c2 * a.istride;
+  cpx<C> *ws = static_cast<cpx<C> *>(a.out) + bi * a.bigdist + c2;
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  const int64_t es = a.n2 * a.istride, n2 = a.n2;
+  const cpx<C> *lo = static_cast<const cpx<C> *>(a.aux0);
+  const cpx<C> *hi = static_cast<const cpx<C> *>(a.aux1);
+  const int shift = a.shift;
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        cpx<C> v = ST::ld(in[(int64_t)e * es]);
+        return BWD ? cswap(v) : v;
+      },
+      [&](int k1, cpx<C> v) {
+        ws[(int64_t)k1 * n2] = cmul(v, split_twiddle(lo, hi, c2 * (int64_t)k1, shift));
+      });
+}
+
+// ---------------------------------------------------------------------------
+// four-step pass B. Row k1 of big transform bi (row = bi*n1 + k1, count =
+// chunk*n1 rows): ws[bi*bigdist + k1*n2 + e], e < n2. Output k2 goes to
+// out[bi*odist + (k1 + n1*k2)*ostride]; the block's TPB consecutive rows are
+// written column-contiguously through LDS (a block may straddle two big
+// transforms).
+// ---------------------------------------------------------------------------
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_row(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t row0 = (int64_t)blockIdx.x * TPB;
+  const int64_t row = row0 + g;
+  const bool valid = row < a.count;
+  const int lper = Eng::lds_per(a.n);
+  cpx<C> *lds_all = dyn_lds<C>();
+  cpx<C> *lds = lds_all + g * lper;
+  const int64_t n1 = a.n1, n2 = a.n2;
+  const cpx<C> *ws = static_cast<const cpx<C> *>(a.in) +
+                     (valid ? (row / n1) * a.bigdist + (row % n1) * n2 : 0);
+  Eng::template run<true>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid, [&](int e) { return ws[e]; },
+      [&](int, cpx<C>) {});
+  // transposed store: thread -> column c = tid % TPB, k2 = tid / TPB + j*TPT
+  const int c = threadIdx.x % TPB;
+  const int64_t rc = row0 + c;
+  if (rc >= a.count) return;
+  const int64_t os = a.ostride;
+  E *out = static_cast<E *>(a.out) + (rc / n1) * a.odist + (rc % n1) * os;
+  const cpx<C> *src = lds_all + c * lper;
+  const int64_t step = n1 * os;
+  for (int k2 = threadIdx.x / TPB; k2 < n2; k2 += TPT) {
+    cpx<C> v = src[lds_pad<Eng::PAD>(k2)];
+    out[k2 * step] = ST::st(BWD ? cswap(v) : v);
+  }
+}
+
+}  // namespace plf

@@ -1,0 +1,34 @@
+// kernels_f16.hip -- f16 storage with FP32 arithmetic.
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_f16(std::vector<KEntry> &r) {
+  using S = half_pair;
+  constexpr int P = P_F16;
+  //                          TPT  TPB  radix chain
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 2);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 16);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    4,  64, 8, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    8,  32, 8, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    8,  32, 16, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,   16,  16, 16, 16);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,   64,   4, 8, 8, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,   64,   4, 16, 16, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,  128,   2, 16, 16, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,  256,   1, 16, 16, 16);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,  512,   1, 16, 16, 16, 2);
+  PLF_STATIC(k_c2c, KK_C2C, S, P, 1024,   1, 16, 16, 16, 4);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P,   16,  16, 16);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P,   64,   4, 16);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P,  256,   1, 16);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P, 1024,   1, 16);
+  PLF_DYNAMIC(k_col, KK_COL, S, P,   64,  16, 16);
+  PLF_DYNAMIC(k_row, KK_ROW, S, P,   64,  16, 16);
+  PLF_DYNAMIC(k_col, KK_COL, S, P,  256,   4, 16);
+  PLF_DYNAMIC(k_row, KK_ROW, S, P,  256,   4, 16);
+}
+
+}  // namespace plf

@@ -1,0 +1,46 @@
+// kernels_f64.hip -- FP64 kernel configurations.
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_f64(std::vector<KEntry> &r) {
+  using S = double;
+  constexpr int P = P_F64;
+  //                          TPT  TPB  radix chain
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 2);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    4,  64, 4, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    4,  64, 8, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    8,  32, 8, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,   16,  16, 8, 4, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,   32,   8, 8, 8, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,   64,   4, 8, 8, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,  128,   2, 8, 8, 4, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,  256,   1, 8, 8, 8, 4);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,  512,   1, 8, 8, 8, 8);
+  PLF_STATIC(k_c2c, KK_C2C, S, P, 1024,   1, 8, 8, 8, 8, 2);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P,   16,  16, 16);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P,   64,   4, 16);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P,  256,   1, 16);
+  PLF_DYNAMIC(k_c2c, KK_C2C, S, P, 1024,   1, 8);
+  // four-step passes: 8 columns of 16 B = 128 B runs
+  PLF_STATIC(k_col, KK_COL, S, P,    8,   8, 8, 8);
+  PLF_STATIC(k_col, KK_COL, S, P,   16,   8, 8, 4, 4);
+  PLF_STATIC(k_col, KK_COL, S, P,   32,   8, 8, 8, 4);
+  PLF_STATIC(k_col, KK_COL, S, P,   64,   8, 8, 8, 8);
+  PLF_STATIC(k_col, KK_COL, S, P,  128,   8, 8, 8, 4, 4);
+  PLF_STATIC(k_col, KK_COL, S, P,  256,   4, 8, 8, 8, 4);
+  PLF_STATIC(k_row, KK_ROW, S, P,    8,   8, 8, 8);
+  PLF_STATIC(k_row, KK_ROW, S, P,   16,   8, 8, 4, 4);
+  PLF_STATIC(k_row, KK_ROW, S, P,   32,   8, 8, 8, 4);
+  PLF_STATIC(k_row, KK_ROW, S, P,   64,   8, 8, 8, 8);
+  PLF_STATIC(k_row, KK_ROW, S, P,  128,   8, 8, 8, 4, 4);
+  PLF_STATIC(k_row, KK_ROW, S, P,  256,   4, 8, 8, 8, 4);
+  PLF_DYNAMIC(k_col, KK_COL, S, P,   64,   8, 16);
+  PLF_DYNAMIC(k_row, KK_ROW, S, P,   64,   8, 16);
+  PLF_DYNAMIC(k_col, KK_COL, S, P,  128,   4, 16);
+  PLF_DYNAMIC(k_row, KK_ROW, S, P,  128,   4, 16);
+}
+
+}  // namespace plf

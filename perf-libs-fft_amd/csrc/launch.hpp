@@ -1,0 +1,102 @@
+// launch.hpp -- launch wrappers and registration helpers for kernels.hpp.
+#pragma once
+
+#include <initializer_list>
+
+#include "kernels.hpp"
+#include "kernels_real.hpp"
+#include "registry.hpp"
+
+namespace plf {
+
+template <void (*K)(KArgs), int BLOCK>
+hipError_t launch_k(const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
+  if (lds > 65536) {
+    static const hipError_t attr = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(K), hipFuncAttributeMaxDynamicSharedMemorySize,
+        160 * 1024);
+    if (attr != hipSuccess) return attr;
+  }
+  hipLaunchKernelGGL(K, dim3((unsigned)grid), dim3(BLOCK), lds, s, a);
+  return hipGetLastError();
+}
+
+inline void add_entry1(std::vector<KEntry> &r, int kind, int prec, int n,
+                       std::initializer_list<int> chain, int tpt, int tpb, int pts,
+                       bool uses_lds, KLaunch fwd) {
+  KEntry e;
+  e.kind = kind;
+  e.prec = prec;
+  e.n = n;
+  e.chain.assign(chain.begin(), chain.end());
+  e.tpt = tpt;
+  e.tpb = tpb;
+  e.pts = pts;
+  e.uses_lds = uses_lds;
+  e.bwd = false;
+  e.launch = fwd;
+  r.push_back(e);
+}
+
+inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
+                      std::initializer_list<int> chain, int tpt, int tpb, int pts,
+                      bool uses_lds, KLaunch fwd, KLaunch bwd) {
+  KEntry e;
+  e.kind = kind;
+  e.prec = prec;
+  e.n = n;
+  e.chain.assign(chain.begin(), chain.end());
+  e.tpt = tpt;
+  e.tpb = tpb;
+  e.pts = pts;
+  e.uses_lds = uses_lds;
+  e.bwd = false;
+  e.launch = fwd;
+  r.push_back(e);
+  e.bwd = true;
+  e.launch = bwd;
+  r.push_back(e);
+}
+
+}  // namespace plf
+
+// Static chain: KERNEL in {k_c2c, k_col, k_row}, KK kind, S storage.
+#define PLF_STATIC(KERNEL, KK, S, PREC, TPT, TPB, ...)                                  \
+  ::plf::add_entry(                                                                    \
+      r, KK, PREC, ::plf::Chain<__VA_ARGS__>::N, {__VA_ARGS__}, TPT, TPB, 0,           \
+      (KK != ::plf::KK_C2C) || (::plf::Chain<__VA_ARGS__>::count > 1),                 \
+      &::plf::launch_k<                                                                \
+          &::plf::KERNEL<S, false, TPB,                                                \
+                         ::plf::SEng<typename ::plf::StorageT<S>::C, TPT, __VA_ARGS__>>, \
+          TPT * TPB>,                                                                  \
+      &::plf::launch_k<                                                                \
+          &::plf::KERNEL<S, true, TPB,                                                 \
+                         ::plf::SEng<typename ::plf::StorageT<S>::C, TPT, __VA_ARGS__>>, \
+          TPT * TPB>)
+
+// Runtime chain (dynamic engine).
+#define PLF_DYNAMIC(KERNEL, KK, S, PREC, TPT, TPB, PTS)                                     \
+  ::plf::add_entry(                                                                        \
+      r, KK, PREC, 0, {}, TPT, TPB, PTS, true,                                             \
+      &::plf::launch_k<                                                                    \
+          &::plf::KERNEL<S, false, TPB, ::plf::DEng<typename ::plf::StorageT<S>::C, TPT, PTS, TPB>>, \
+          TPT * TPB>,                                                                      \
+      &::plf::launch_k<                                                                    \
+          &::plf::KERNEL<S, true, TPB, ::plf::DEng<typename ::plf::StorageT<S>::C, TPT, PTS, TPB>>, \
+          TPT * TPB>)
+
+// Single-direction kinds (real transforms): one entry, bwd == false.
+#define PLF_STATIC1(KERNEL, KK, S, PREC, TPT, TPB, ...)                                 \
+  ::plf::add_entry1(                                                                   \
+      r, KK, PREC, ::plf::Chain<__VA_ARGS__>::N, {__VA_ARGS__}, TPT, TPB, 0, true,     \
+      &::plf::launch_k<                                                                \
+          &::plf::KERNEL<S, false, TPB,                                                \
+                         ::plf::SEng<typename ::plf::StorageT<S>::C, TPT, __VA_ARGS__>>, \
+          TPT * TPB>)
+
+#define PLF_DYNAMIC1(KERNEL, KK, S, PREC, TPT, TPB, PTS)                                    \
+  ::plf::add_entry1(                                                                       \
+      r, KK, PREC, 0, {}, TPT, TPB, PTS, true,                                             \
+      &::plf::launch_k<                                                                    \
+          &::plf::KERNEL<S, false, TPB, ::plf::DEng<typename ::plf::StorageT<S>::C, TPT, PTS, TPB>>, \
+          TPT * TPB>)

@@ -63,6 +63,33 @@ struct PlanError {
 // Builds the node for a validated problem; returns null (and a reason) when
 // no GPU implementation exists.
 std::unique_ptr<Node> build_node(const Problem &p, std::string *why);
+std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why);
+std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why);
+
+// ---- planner internals shared by the node builders ----------------------
+// A kernel choice for one engine length: the registry entry plus the chain.
+struct Choice {
+  const KEntry *e = nullptr;
+  std::vector<int> chain;
+};
+// Compile-time chain first (registry order = preference), else the dynamic
+// entry with the fewest threads that fits; e == null if none.
+Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div = 0);
+std::vector<int> dynamic_chain(int64_t n);
+std::string chain_str(const Choice &c);
+// Upload a FP64 table (interleaved re/im) in the compute precision.
+hipError_t upload_table(DevMem &dst, const std::vector<double> &v, int prec);
+
+// Device-side engine data for a chain: Stockham twiddles and, for the
+// dynamic engine, its descriptor.
+struct EngineData {
+  DevMem tw, desc;
+  hipError_t init(const Choice &c, int64_t len);
+  const GenericDesc *d() const { return static_cast<const GenericDesc *>(desc.p); }
+};
+
+// W_n^k (forward sign) for k in [0, count), interleaved re/im, FP64.
+std::vector<double> root_table(int64_t n, int64_t count);
 
 // Twiddle table of a Stockham radix chain, FORWARD sign, FP64:
 // for stage s >= 1 with radix R and L = product of earlier radices,

@@ -19,27 +19,65 @@ inline const char *prec_name(int p) {
   return p == P_F32 ? "f32" : p == P_F64 ? "f64" : p == P_F16 ? "f16" : "bf16";
 }
 
-using FusedLaunch = hipError_t (*)(const FusedArgs &, int64_t grid, hipStream_t);
+// Kernel families (see kernels.hpp).
+enum KKind : int {
+  KK_C2C = 0,
+  KK_COL = 1,
+  KK_ROW = 2,
+  KK_R2C_E = 3,  // R2C, even n, engine length n/2
+  KK_C2R_E = 4,  // C2R, even n, engine length n/2
+  KK_R2C_F = 5,  // R2C, full-length engine
+  KK_C2R_F = 6,  // C2R, full-length engine
+};
+inline const char *kkind_name(int k) {
+  static const char *names[] = {"c2c", "col", "row", "r2c_half", "c2r_half", "r2c_full", "c2r_full"};
+  return (k >= 0 && k <= 6) ? names[k] : "?";
+}
 
-// One compiled fused Stockham kernel: length n = product of `chain`, TPT
-// threads per transform, TPB transforms per block.
-struct FusedEntry {
-  int n;
+using KLaunch = hipError_t (*)(const KArgs &, int64_t grid, size_t lds, hipStream_t);
+
+// One compiled kernel. n > 0: compile-time chain of length n; n == 0:
+// runtime chain (dynamic engine) with at most pts points per thread.
+struct KEntry {
+  int kind;
   int prec;
   bool bwd;
-  int tpt, tpb, padshift;
+  int n;
   std::vector<int> chain;
-  size_t lds_bytes;
-  int block;
-  FusedLaunch launch;
+  int tpt, tpb, pts;
+  bool uses_lds;
+  KLaunch launch;
   std::string name() const;
+  // LDS bytes of one block for engine length len
+  size_t lds_bytes(int64_t len) const {
+    if (!uses_lds) return 0;
+    return (size_t)tpb * (size_t)(len + (len >> 5) + 1) * 2 * prec_compute_bytes(prec);
+  }
+  // can the dynamic engine run this radix chain within pts points/thread
+  // (and with the butterflies this configuration compiles)?
+  bool fits(int64_t len, const std::vector<int> &ch) const;
+  // dynamic entries: radix 17/19/23 butterflies present (<= 256-thread blocks)
+  bool big_primes() const { return n == 0 && tpt <= 256 && tpb <= 16 && tpt * tpb <= 256; }
 };
 
-const std::vector<FusedEntry> &fused_registry();
+const std::vector<KEntry> &kernel_registry();
+void register_kernels_f32(std::vector<KEntry> &r);
+void register_kernels_f32_4s(std::vector<KEntry> &r);
+void register_kernels_f32_real(std::vector<KEntry> &r);
+void register_kernels_f64_real(std::vector<KEntry> &r);
+void register_kernels_h_real(std::vector<KEntry> &r);
+void register_kernels_f64(std::vector<KEntry> &r);
+void register_kernels_f16(std::vector<KEntry> &r);
+void register_kernels_bf16(std::vector<KEntry> &r);
 
-void register_fused_f32(std::vector<FusedEntry> &r);
-void register_fused_f64(std::vector<FusedEntry> &r);
-void register_fused_f16(std::vector<FusedEntry> &r);
-void register_fused_bf16(std::vector<FusedEntry> &r);
+// Radices with a butterfly in the dynamic engine.
+inline bool dynamic_radix_ok(int r) {
+  switch (r) {
+    case 2: case 3: case 4: case 5: case 7: case 8: case 11: case 13: case 16:
+    case 17: case 19: case 23:
+      return true;
+  }
+  return false;
+}
 
 }  // namespace plf

@@ -158,3 +158,64 @@ def test_adjoint_c2c(torch_cuda):
     lhs = np.vdot(from_dev(torch, fx), y)
     rhs = np.vdot(x, from_dev(torch, ay))
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+# Mixed-radix lengths served by the runtime-chain engine (prime factors
+# <= 23), including 15015 = 3*5*7*11*13 of the prime/mixed-radix config.
+MIXED = [3, 5, 6, 7, 9, 10, 12, 15, 17, 19, 23, 24, 25, 30, 36, 49, 60, 96, 100, 120,
+         125, 243, 360, 529, 1000, 1001, 1058, 3072, 6000, 10240, 15015]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", MIXED)
+def test_c2c_mixed_radix_vs_oracle(torch_cuda, orc, prec, n):
+    torch = torch_cuda
+    hm = max(1, min(24, (1 << 17) // n)) + 1
+    x = rng.complex_normal(n + 1, n * hm).astype(CPLX_NP[prec])
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    for direction in (-1, 1):
+        plan = P.Plan("c2c", n, direction, prec, howmany=hm)
+        plan.execute(xd, yd)
+        y = from_dev(torch, yd)
+        ref = orc.c2c_batched(x.astype(np.complex128), n, direction, hm, 1, n, 1, n, n * hm)
+        assert relerr(y, ref) < TOL[prec], plan.describe()
+
+
+# Lengths beyond one LDS tile: the four-step column/row passes.
+LARGE = [(32768, "f32"), (65536, "f32"), (131072, "f32"), (1 << 20, "f32"),
+         (16384, "f64"), (65536, "f64"), (1 << 20, "f64"), (3 * 65536, "f32"),
+         (100000, "f32")]
+
+
+@pytest.mark.parametrize("n,prec", LARGE)
+def test_c2c_four_step_vs_oracle(torch_cuda, orc, n, prec):
+    torch = torch_cuda
+    hm = 3
+    x = rng.complex_normal(n + 7, n * hm).astype(CPLX_NP[prec])
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    plan = P.Plan("c2c", n, -1, prec, howmany=hm)
+    assert "fourstep" in plan.describe()
+    plan.execute(xd, yd)
+    y = from_dev(torch, yd)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
+    assert relerr(y, ref) < TOL[prec], plan.describe()
+    # in place, backward, with the round trip returning n*x
+    plan_b = P.Plan("c2c", n, 1, prec, howmany=hm, may_alias=True)
+    plan_b.execute(yd, yd)
+    z = from_dev(torch, yd)
+    assert relerr(z / n, x) < TOL[prec]
+
+
+def test_four_step_strided_batch(torch_cuda, orc):
+    """Interleaved members (istride = howmany) through the four-step path."""
+    torch = torch_cuda
+    n, hm = 65536, 4
+    x = rng.complex_normal(5, n * hm).astype(np.complex64)
+    xd = to_dev(torch, x)
+    yd = torch.zeros_like(xd)
+    plan = P.Plan("c2c", n, -1, "f32", hm, istride=hm, idist=1, ostride=1, odist=n)
+    plan.execute(xd, yd)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, hm, 1, 1, n, n * hm)
+    assert relerr(from_dev(torch, yd), ref) < TOL["f32"]

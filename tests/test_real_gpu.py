@@ -1,0 +1,159 @@
+"""R2C / C2R parity on the GPU against the FP64 oracle.
+
+Layouts replay the reference's tests: the strided R2C/C2R table of
+test/r2c_c2r_1d_batched.cpp:40-116, the in-place batched round trip of
+test/plfft_interface_test.cpp:360-412, the n = 2..128 sweep of
+test/simple_test.cpp:14-49 (X[0] = sum x, round trip n*x), and the
+differentiable-transform layout (n=4096, input stride 2, distance 2n+64).
+"""
+import numpy as np
+import pytest
+
+import plfft_amd as P
+from plfft_amd import rng
+from gpu_util import TOL, from_dev, relerr, span, to_dev
+
+pytestmark = pytest.mark.gpu
+
+RNP = {"f32": np.float32, "f64": np.float64}
+CNP = {"f32": np.complex64, "f64": np.complex128}
+
+
+def _r2c(torch, orc, prec, n, hm, is_, id_, os_, od_):
+    isz, osz = span(n, is_, hm, id_), span(n // 2 + 1, os_, hm, od_)
+    x = rng.normal_f64(n * 3 + hm, isz).astype(RNP[prec])
+    xd = to_dev(torch, x)
+    yd = torch.zeros(osz, dtype=getattr(torch, "complex64" if prec == "f32" else "complex128"),
+                     device="cuda")
+    plan = P.Plan("r2c", n, -1, prec, hm, is_, id_, os_, od_)
+    plan.execute(xd, yd)
+    ref = orc.r2c_batched(x.astype(np.float64), n, hm, is_, id_, os_, od_, osz)
+    return x, from_dev(torch, yd), ref, plan
+
+
+def _c2r(torch, orc, prec, n, hm, is_, id_, os_, od_, X):
+    osz = span(n, os_, hm, od_)
+    Xd = to_dev(torch, X.astype(CNP[prec]))
+    yd = torch.zeros(osz, dtype=getattr(torch, "float32" if prec == "f32" else "float64"),
+                     device="cuda")
+    plan = P.Plan("c2r", n, 1, prec, hm, is_, id_, os_, od_)
+    plan.execute(Xd, yd)
+    ref = orc.c2r_batched(X.astype(np.complex128), n, hm, is_, id_, os_, od_, osz)
+    return from_dev(torch, yd), ref, plan
+
+
+SIZES = [1, 2, 3, 4, 5, 6, 7, 8, 9, 12, 15, 16, 17, 30, 31, 46, 64, 77, 100, 128, 256, 500,
+         1000, 1024, 2048, 4096, 6000, 8192, 16384]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", SIZES)
+def test_r2c_c2r_contiguous(torch_cuda, orc, prec, n):
+    torch = torch_cuda
+    hm = max(1, min(16, (1 << 16) // n)) + 1
+    m = n // 2 + 1
+    x, y, ref, plan = _r2c(torch, orc, prec, n, hm, 1, n, 1, m)
+    assert relerr(y, ref) < TOL[prec], plan.describe()
+    # C2R of the spectrum: the round trip returns n*x
+    z, zref, plan2 = _c2r(torch, orc, prec, n, hm, 1, m, 1, n, ref)
+    assert relerr(z, zref) < TOL[prec], plan2.describe()
+    assert relerr(z / n, x.astype(np.float64)) < TOL[prec]
+
+
+def test_c2r_ignores_imaginary_dc_and_nyquist(torch_cuda, orc):
+    torch = torch_cuda
+    for n in (8, 9, 4096):
+        m = n // 2 + 1
+        X = rng.complex_normal(n, m * 2).astype(np.complex128)
+        z, zref, _ = _c2r(torch, orc, "f64", n, 2, 1, m, 1, n, X)
+        X2 = X.reshape(2, m).copy()
+        X2[:, 0] = X2[:, 0].real
+        if n % 2 == 0:
+            X2[:, -1] = X2[:, -1].real
+        z2, _, _ = _c2r(torch, orc, "f64", n, 2, 1, m, 1, n, X2.ravel())
+        assert relerr(z, zref) < 1e-13
+        assert np.allclose(z, z2, atol=1e-12 * n)
+
+
+# test/r2c_c2r_1d_batched.cpp:40-60 (n, howmany, istride, idist, ostride, odist)
+STRIDED = [(22, 3, 3, 88, 3, 90), (46, 4, 2, 100, 2, 92), (12, 5, 1, 12, 1, 12),
+           (128, 6, 5, 650, 2, 256), (77, 7, 4, 308, 3, 231)]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("case", STRIDED)
+def test_r2c_c2r_strided_round_trip(torch_cuda, orc, prec, case):
+    torch = torch_cuda
+    n, hm, is_, id_, os_, od_ = case
+    x, y, ref, plan = _r2c(torch, orc, prec, n, hm, is_, id_, os_, od_)
+    assert relerr(y, ref) < TOL[prec], plan.describe()
+    # inverse with the layouts swapped, as the reference test does
+    z, zref, _ = _c2r(torch, orc, prec, n, hm, os_, od_, is_, id_, ref)
+    assert relerr(z, zref) < TOL[prec]
+    for b in range(hm):
+        idx = b * id_ + is_ * np.arange(n)
+        assert relerr(z[idx] / n, x[idx].astype(np.float64)) < TOL[prec]
+
+
+@pytest.mark.parametrize("n", [4, 5])
+def test_in_place_batched_round_trip_host(n):
+    """test/plfft_interface_test.cpp:360-412 on host memory (staged)."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    hm, rspan, cspan = 2, 6, 3
+    data = np.zeros(hm * rspan, np.float32)
+    for b in range(hm):
+        for e in range(n):
+            data[b * rspan + e] = b * 10 + e + 1
+    expected = data * n
+    r2c = P.Plan("r2c", n, -1, "f32", hm, 1, rspan, 1, cspan, may_alias=True)
+    c2r = P.Plan("c2r", n, 1, "f32", hm, 1, cspan, 1, rspan, may_alias=True)
+    r2c.execute(data, data)
+    c2r.execute(data, data)
+    for b in range(hm):
+        for e in range(n):
+            i = b * rspan + e
+            assert abs(data[i] - expected[i]) < 1e-4
+
+
+@pytest.mark.parametrize("n", [4096, 1000])
+def test_in_place_device(torch_cuda, orc, n):
+    torch = torch_cuda
+    hm = 9
+    m = n // 2 + 1
+    rdist = 2 * m  # padded real rows, FFTW-style in-place layout
+    x = rng.normal_f64(1, hm * rdist).astype(np.float32)
+    d = to_dev(torch, x)
+    P.Plan("r2c", n, -1, "f32", hm, 1, rdist, 1, m, may_alias=True).execute(d, d)
+    Y = from_dev(torch, d).view(np.complex64)
+    ref = orc.r2c_batched(x.astype(np.float64), n, hm, 1, rdist, 1, m, hm * m)
+    assert relerr(Y, ref) < TOL["f32"]
+    P.Plan("c2r", n, 1, "f32", hm, 1, m, 1, rdist, may_alias=True).execute(d, d)
+    z = from_dev(torch, d)
+    for b in range(hm):
+        assert relerr(z[b * rdist:b * rdist + n] / n, x[b * rdist:b * rdist + n]) < TOL["f32"]
+
+
+def test_simple_sweep_first_output_is_sum(torch_cuda):
+    """test/simple_test.hpp:16-95: X[0] = sum x, C2R[0] = Hermitian sum."""
+    torch = torch_cuda
+    for n in range(2, 129):
+        x = (np.arange(n) % 7 - 3).astype(np.float32)
+        X = torch.zeros(n // 2 + 1, dtype=torch.complex64, device="cuda")
+        P.Plan("r2c", n, -1, "f32").execute(to_dev(torch, x), X)
+        Xh = from_dev(torch, X)
+        assert abs(Xh[0].real - x.sum()) <= 1e-5 * n and abs(Xh[0].imag) <= 1e-5
+        y = torch.zeros(n, dtype=torch.float32, device="cuda")
+        P.Plan("c2r", n, 1, "f32").execute(X, y)
+        yh = from_dev(torch, y)
+        herm = Xh[0].real + 2 * Xh[1:(n + 1) // 2].real.sum() + (Xh[n // 2].real if n % 2 == 0 else 0)
+        assert abs(yh[0] - herm) <= 1e-4 * n
+
+
+def test_differentiable_layout_r2c(torch_cuda, orc):
+    """n=4096, input stride 2, distance 2n+64 (reduced batch)."""
+    torch = torch_cuda
+    n, hm, is_, id_ = 4096, 16, 2, 2 * 4096 + 64
+    x, y, ref, plan = _r2c(torch, orc, "f32", n, hm, is_, id_, 1, n // 2 + 1)
+    assert relerr(y, ref) < TOL["f32"], plan.describe()

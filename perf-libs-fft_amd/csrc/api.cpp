@@ -261,6 +261,12 @@ extern "C" plfft_status_t plfft_plan_create_from_config(plfft_plan_t **plan, con
     default: return PLFFT_INVALID_DATA_TYPE;
   }
   // fixed-point R2R has no implementation (src/plfft.cpp:248-260)
+  // DCT-I is undefined for n == 1 (docs/plfft.md:208-209)
+  if (config->transform_kind == TRANSFORM_R2R && config->payload == PLFFT_R2R_DCT_1 &&
+      config->n == 1) {
+    plf::set_last_error("DCT-I needs n > 1");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
   if (p.fixed_bits && config->transform_kind == TRANSFORM_R2R) {
     plf::set_last_error("fixed-point real-to-real transforms are not supported");
     return PLFFT_PLAN_CREATION_FAILED;
@@ -477,14 +483,41 @@ static const plf::Node *adjoint_node(const plfft_plan_t *pl) {
     plf::Problem q = pl->prob;
     std::swap(q.istride, q.ostride);
     std::swap(q.idist, q.odist);
-    if (q.kind == plf::K_C2C) {
-      q.dir = -q.dir;
+    bool ok = true;
+    switch (q.kind) {
+      case plf::K_C2C:  // adjoint of the unnormalized DFT: opposite direction
+        q.dir = -q.dir;
+        break;
+      case plf::K_R2C:  // C2R with the interior bins halved
+        q.kind = plf::K_C2R;
+        q.dir = PLFFT_BACKWARD;
+        q.adjoint = 1;
+        break;
+      case plf::K_C2R:  // R2C with the interior bins doubled
+        q.kind = plf::K_R2C;
+        q.dir = PLFFT_FORWARD;
+        q.adjoint = 1;
+        break;
+      case plf::K_R2R:  // DCT-II <-> DCT-III, DST-II <-> DST-III with end weights
+        switch (q.r2r_kind) {
+          case PLFFT_R2R_DCT_2: q.r2r_kind = PLFFT_R2R_DCT_3; break;
+          case PLFFT_R2R_DCT_3: q.r2r_kind = PLFFT_R2R_DCT_2; break;
+          case PLFFT_R2R_DST_2: q.r2r_kind = PLFFT_R2R_DST_3; break;
+          case PLFFT_R2R_DST_3: q.r2r_kind = PLFFT_R2R_DST_2; break;
+          default: ok = false;
+        }
+        q.adjoint = 1;
+        break;
+      default:
+        ok = false;
+    }
+    if (ok) {
       std::string why;
       DeviceGuard guard(pl->device);
       pl->adj = plf::build_node(q, &why);
       if (!pl->adj) plf::set_last_error(why);
     } else {
-      plf::set_last_error("adjoint not implemented for this transform kind yet");
+      plf::set_last_error("no adjoint for this transform kind");
     }
   }
   return pl->adj.get();

@@ -1,0 +1,306 @@
+// conv_plan.cpp -- plan nodes for lengths with large prime factors.
+//
+// The reference tries Rader before Bluestein (src/planner/planner.cpp:178-179);
+// here a prime n whose n-1 has an LDS kernel runs fused Rader, other lengths
+// run fused Bluestein when the padded length M >= 2n-1 fits one LDS tile,
+// and the rest run a multi-kernel chirp-z around the four-step transform.
+// The convolution kernel B = DFT(b)/M is computed once at plan time in FP64
+// on the GPU with this library's own C2C plan, then rounded to the compute
+// precision (the reference likewise transforms b with its own plan,
+// bluestein.hpp:100-110, rader.hpp:79-90).
+#include <algorithm>
+#include <cstdlib>
+#include <sstream>
+
+#include "plan.hpp"
+
+namespace plf {
+
+static int64_t spf(int64_t n) {
+  if (n % 2 == 0) return 2;
+  for (int64_t d = 3; d * d <= n; d += 2)
+    if (n % d == 0) return d;
+  return n;
+}
+
+static int64_t group_generator(int64_t n) {
+  // smallest g whose powers visit every nonzero residue (rader_generator.cpp:13-33)
+  std::vector<char> seen((size_t)n);
+  for (int64_t g = 2; g < n; ++g) {
+    std::fill(seen.begin(), seen.end(), 0);
+    int64_t x = 1;
+    bool ok = true;
+    for (int64_t i = 1; i < n; ++i) {
+      if (seen[(size_t)x]) {
+        ok = false;
+        break;
+      }
+      seen[(size_t)x] = 1;
+      x = x * g % n;
+    }
+    if (ok) return g;
+  }
+  return 0;
+}
+
+// In-place forward DFT of v (interleaved FP64) on the GPU.
+static hipError_t dft_f64_gpu(std::vector<double> &v, int64_t M) {
+  Problem q;
+  q.kind = K_C2C;
+  q.dir = -1;
+  q.prec = P_F64;
+  q.n = M;
+  q.howmany = 1;
+  q.may_alias = true;
+  std::string why;
+  std::unique_ptr<Node> node = build_c2c_node(q, &why);
+  if (!node) return hipErrorInvalidValue;
+  DevMem buf, ws;
+  hipError_t err = buf.upload(v.data(), v.size() * sizeof(double));
+  if (err == hipSuccess) err = ws.alloc(node->workspace_bytes());
+  if (err == hipSuccess) err = node->enqueue(buf.p, buf.p, ws.p, nullptr);
+  if (err == hipSuccess) err = hipStreamSynchronize(nullptr);
+  if (err == hipSuccess)
+    err = hipMemcpy(v.data(), buf.p, v.size() * sizeof(double), hipMemcpyDeviceToHost);
+  return err;
+}
+
+// Chirp c_k = exp(-i pi (k^2 mod 2n)/n), phase in exact integers
+// (bluestein.hpp:66-79).
+static std::vector<double> chirp_table(int64_t n) {
+  std::vector<double> c(2 * (size_t)n);
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t ph = (int64_t)(((unsigned __int128)k * (unsigned __int128)k) %
+                                 (unsigned __int128)(2 * n));
+    double re, im;
+    unit_root(ph, 2 * n, &re, &im);
+    c[2 * k] = re;
+    c[2 * k + 1] = -im;
+  }
+  return c;
+}
+
+// B = DFT_M(b)/M with b_k = b_{M-k} = conj(c_k), 0 <= k < n (bluestein.hpp:81-98).
+static hipError_t bluestein_kernel(int64_t n, int64_t M, const std::vector<double> &c,
+                                   std::vector<double> *B) {
+  B->assign(2 * (size_t)M, 0.0);
+  const double s = 1.0 / (double)M;
+  for (int64_t k = 0; k < n; ++k) {
+    const double re = c[2 * k] * s, im = -c[2 * k + 1] * s;
+    (*B)[2 * k] = re;
+    (*B)[2 * k + 1] = im;
+    if (k) {
+      (*B)[2 * (M - k)] = re;
+      (*B)[2 * (M - k) + 1] = im;
+    }
+  }
+  return dft_f64_gpu(*B, M);
+}
+
+// ---------------------------------------------------------------------------
+class FusedConvNode : public Node {
+ public:
+  FusedConvNode(bool rader, Choice c, int64_t n, int64_t len, int prec, int64_t batch,
+                int64_t is, int64_t id, int64_t os, int64_t od)
+      : rader_(rader), c_(std::move(c)), n_(n), len_(len), prec_(prec), batch_(batch), is_(is),
+        id_(id), os_(os), od_(od) {}
+  hipError_t init() {
+    hipError_t err = eng_.init(c_, len_);
+    if (err != hipSuccess) return err;
+    std::vector<double> B;
+    if (rader_) {
+      const int64_t m = len_;
+      const int64_t g = group_generator(n_);
+      std::vector<int> perm((size_t)m);
+      std::vector<double> b(2 * (size_t)m);
+      int64_t x = 1;
+      for (int64_t k = 0; k < m; ++k) {
+        perm[(size_t)k] = (int)x;
+        double re, im;
+        unit_root(x, n_, &re, &im);  // W_n^{perm[k]}, forward sign
+        b[2 * ((m - k) % m)] = re / (double)m;
+        b[2 * ((m - k) % m) + 1] = -im / (double)m;
+        x = x * g % n_;
+      }
+      err = dft_f64_gpu(b, m);
+      if (err == hipSuccess) err = upload_table(aux0_, b, prec_);
+      if (err == hipSuccess) err = aux1_.upload(perm.data(), perm.size() * sizeof(int));
+      gen_ = g;
+    } else {
+      std::vector<double> c = chirp_table(n_);
+      err = bluestein_kernel(n_, len_, c, &B);
+      if (err == hipSuccess) err = upload_table(aux0_, B, prec_);
+      if (err == hipSuccess) err = upload_table(aux1_, c, prec_);
+    }
+    return err;
+  }
+  hipError_t enqueue(const void *in, void *out, void *, hipStream_t s) const override {
+    if (batch_ <= 0) return hipSuccess;
+    KArgs a{};
+    a.in = in;
+    a.out = out;
+    a.tw = eng_.tw.p;
+    a.desc = eng_.d();
+    a.aux0 = aux0_.p;
+    a.aux1 = aux1_.p;
+    a.count = batch_;
+    a.istride = is_;
+    a.idist = id_;
+    a.ostride = os_;
+    a.odist = od_;
+    a.n = len_;
+    a.n1 = n_;
+    const int64_t grid = (batch_ + c_.e->tpb - 1) / c_.e->tpb;
+    return c_.e->launch(a, grid, c_.e->lds_bytes(len_), s);
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(" << (rader_ ? "rader" : "bluestein") << " n=" << n_ << (rader_ ? " m=" : " M=")
+      << len_;
+    if (rader_) o << " g=" << gen_;
+    o << " " << chain_str(c_) << " batch=" << batch_ << ")";
+    return o.str();
+  }
+
+ private:
+  bool rader_;
+  Choice c_;
+  int64_t n_, len_;
+  int prec_;
+  int64_t batch_, is_, id_, os_, od_;
+  EngineData eng_;
+  DevMem aux0_, aux1_;
+  int64_t gen_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Chirp-z around a length-M C2C node (four-step) for n beyond one LDS tile.
+// ---------------------------------------------------------------------------
+class BigBluesteinNode : public Node {
+ public:
+  BigBluesteinNode(int64_t n, int64_t M, int prec, bool bwd, int64_t batch, int64_t is,
+                   int64_t id, int64_t os, int64_t od, int64_t chunk)
+      : n_(n), M_(M), prec_(prec), bwd_(bwd), batch_(batch), is_(is), id_(id), os_(os), od_(od),
+        chunk_(chunk) {}
+  hipError_t init(std::string *why) {
+    Problem q;
+    q.kind = K_C2C;
+    q.dir = -1;
+    q.prec = prec_compute_bytes(prec_) == 8 ? P_F64 : P_F32;
+    q.n = M_;
+    q.howmany = chunk_;
+    q.istride = q.ostride = 1;
+    q.idist = q.odist = M_;
+    q.may_alias = true;
+    inner_ = build_c2c_node(q, why);
+    if (!inner_) return hipErrorInvalidValue;
+    std::vector<double> c = chirp_table(n_), B;
+    hipError_t err = bluestein_kernel(n_, M_, c, &B);
+    if (err == hipSuccess) err = upload_table(B_, B, prec_);
+    if (err == hipSuccess) err = upload_table(c_, c, prec_);
+    return err;
+  }
+  size_t buf_bytes() const { return (size_t)chunk_ * (size_t)M_ * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return buf_bytes() + inner_->workspace_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    char *buf = static_cast<char *>(work);
+    void *iws = buf + buf_bytes();
+    const size_t eb = 2 * (size_t)prec_real_bytes(prec_);
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk_) {
+      const int64_t cb = std::min(chunk_, batch_ - b0);
+      KArgs a{};
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * eb;
+      a.out = buf;
+      a.aux0 = B_.p;
+      a.aux1 = c_.p;
+      a.count = cb;
+      a.istride = is_;
+      a.idist = id_;
+      a.ostride = os_;
+      a.odist = od_;
+      a.n = M_;
+      a.n1 = n_;
+      hipError_t err = launch_chirp_in(prec_, bwd_, a, s);
+      if (err == hipSuccess) err = inner_->enqueue(buf, buf, iws, s);
+      if (err == hipSuccess) err = launch_pointwise_swap(prec_, a, s);
+      if (err == hipSuccess) err = inner_->enqueue(buf, buf, iws, s);
+      if (err != hipSuccess) return err;
+      a.in = buf;
+      a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * eb;
+      err = launch_chirp_out(prec_, bwd_, a, s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(chirpz n=" << n_ << " M=" << M_ << " chunk=" << chunk_ << " batch=" << batch_ << " "
+      << inner_->describe() << ")";
+    return o.str();
+  }
+
+ private:
+  int64_t n_, M_;
+  int prec_;
+  bool bwd_;
+  int64_t batch_, is_, id_, os_, od_, chunk_;
+  std::unique_ptr<Node> inner_;
+  DevMem B_, c_;
+};
+
+static bool smooth7(int64_t v) {
+  for (int p : {2, 3, 5, 7})
+    while (v % p == 0) v /= p;
+  return v == 1;
+}
+
+std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
+  const bool bwd = p.dir > 0;
+  const int64_t id = p.howmany > 1 ? p.idist : 0;
+  const int64_t od = p.howmany > 1 ? p.odist : 0;
+  auto finish = [&](std::unique_ptr<FusedConvNode> node) -> std::unique_ptr<Node> {
+    hipError_t err = node->init();
+    if (err != hipSuccess) {
+      *why = std::string("convolution table setup failed: ") + hipGetErrorString(err);
+      return nullptr;
+    }
+    return node;
+  };
+  // Rader for primes whose n-1 has an LDS kernel
+  if (p.n > 2 && spf(p.n) == p.n) {
+    Choice c = choose(KK_RADER, p.n - 1, p.prec, bwd);
+    if (c.e)
+      return finish(std::make_unique<FusedConvNode>(true, c, p.n, p.n - 1, p.prec, p.howmany,
+                                                    p.istride, id, p.ostride, od));
+  }
+  // fused Bluestein: a power of two with a compiled chain, else a 7-smooth length
+  int64_t P2 = 1;
+  while (P2 < 2 * p.n - 1) P2 *= 2;
+  Choice c = choose(KK_BLUE, P2, p.prec, bwd);
+  int64_t M = P2;
+  if (!c.e || !c.e->n) {
+    int64_t S = 2 * p.n - 1;
+    while (!smooth7(S)) ++S;
+    Choice cs = choose(KK_BLUE, S, p.prec, bwd);
+    if (cs.e) {
+      c = cs;
+      M = S;
+    }
+  }
+  if (c.e)
+    return finish(std::make_unique<FusedConvNode>(false, c, p.n, M, p.prec, p.howmany, p.istride,
+                                                  id, p.ostride, od));
+  // chirp-z around the four-step transform
+  const int64_t per = P2 * 2 * prec_compute_bytes(p.prec);
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(p.howmany, (int64_t(64) << 20) / per));
+  auto node = std::make_unique<BigBluesteinNode>(p.n, P2, p.prec, bwd, p.howmany, p.istride, id,
+                                                 p.ostride, od, chunk);
+  hipError_t err = node->init(why);
+  if (err != hipSuccess) {
+    if (why->empty()) *why = std::string("chirp-z setup failed: ") + hipGetErrorString(err);
+    return nullptr;
+  }
+  return node;
+}
+
+}  // namespace plf

@@ -37,7 +37,7 @@ struct StaticEngine {
   static constexpr int LDS_PER = lds_padded<PAD>(N);
   static constexpr bool USES_LDS = NS > 1;
 
-  template <int SI, bool TO_LDS, class Load, class Store>
+  template <int SI, bool TO_LDS, bool LDS_IN, class Load, class Store>
   static __device__ __forceinline__ void stage(cpx<C> *lds, const cpx<C> *__restrict__ tw,
                                                int t, bool valid, Load &load, Store &store) {
     constexpr int R = Ch::radix(SI);
@@ -53,7 +53,7 @@ struct StaticEngine {
       const bool act = (NB % TPT == 0) || (b < NB);
 #pragma unroll
       for (int m = 0; m < R; ++m) {
-        if constexpr (FIRST) {
+        if constexpr (FIRST && !LDS_IN) {
           v[k][m] = (act && valid) ? load(b + m * NB) : cpx<C>{C(0), C(0)};
         } else {
           v[k][m] = act ? lds[lds_pad<PAD>(b + m * NB)] : cpx<C>{C(0), C(0)};
@@ -74,7 +74,7 @@ struct StaticEngine {
       }
       Dft<R, C>::run(v[k]);
     }
-    if constexpr (!FIRST && (!LAST || TO_LDS)) __syncthreads();  // all reads done
+    if constexpr ((!FIRST || LDS_IN) && (!LAST || TO_LDS)) __syncthreads();  // all reads done
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       const int b = t + k * TPT;
@@ -99,14 +99,15 @@ struct StaticEngine {
       }
     }
     if constexpr (!LAST || TO_LDS) __syncthreads();
-    if constexpr (!LAST) stage<SI + 1, TO_LDS>(lds, tw, t, valid, load, store);
+    if constexpr (!LAST) stage<SI + 1, TO_LDS, false>(lds, tw, t, valid, load, store);
   }
 
-  // Results stay in LDS (natural order, lds[lds_pad(e)]) when TO_LDS.
-  template <bool TO_LDS, class Load, class Store>
+  // Results stay in LDS (natural order, lds[lds_pad(e)]) when TO_LDS. With
+  // LDS_IN the input is taken from LDS (natural order) instead of `load`.
+  template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
   static __device__ __forceinline__ void run(cpx<C> *lds, const cpx<C> *tw, int t, bool valid,
                                              Load &&load, Store &&store) {
-    stage<0, TO_LDS>(lds, tw, t, valid, load, store);
+    stage<0, TO_LDS, LDS_IN>(lds, tw, t, valid, load, store);
   }
 };
 
@@ -122,12 +123,14 @@ struct DynamicEngine {
   template <int R, bool TO_LDS, class Load, class Store>
   static __device__ __forceinline__ void stage(const GenericDesc *__restrict__ d, int s,
                                                cpx<C> *lds, const cpx<C> *__restrict__ tw,
-                                               int t, bool valid, Load &load, Store &store) {
+                                               int t, bool valid, bool lds_in, Load &load,
+                                               Store &store) {
     constexpr int KMAX = (PTS + R - 1) / R;
     const int n = d->n;
     const int NB = n / R;
     const int L = d->L[s];
     const bool first = (s == 0);
+    const bool from_global = first && !lds_in;
     const bool last = (s == d->ns - 1);
     const cpx<C> *tws = tw + d->twoff[s];
     cpx<C> v[KMAX][R];
@@ -139,7 +142,7 @@ struct DynamicEngine {
       for (int m = 0; m < R; ++m) {
         cpx<C> e = {C(0), C(0)};
         if (act) {
-          if (first) {
+          if (from_global) {
             if (valid) e = load(b + m * NB);
           } else {
             e = lds[lds_pad<PAD>(b + m * NB)];
@@ -158,7 +161,7 @@ struct DynamicEngine {
       }
       Dft<R, C>::run(v[k]);
     }
-    if (!first && (!last || TO_LDS)) __syncthreads();
+    if (!from_global && (!last || TO_LDS)) __syncthreads();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
       const int b = t + k * TPT;
@@ -182,25 +185,26 @@ struct DynamicEngine {
     if (!last || TO_LDS) __syncthreads();
   }
 
-  template <bool TO_LDS, class Load, class Store>
+  template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
   static __device__ __forceinline__ void run(const GenericDesc *__restrict__ d, cpx<C> *lds,
                                              const cpx<C> *tw, int t, bool valid, Load &&load,
                                              Store &&store) {
     const int ns = d->ns;
     if (ns == 0) {  // n == 1: identity
       if (TO_LDS) {
-        if (t == 0 && valid) lds[0] = load(0);
+        if (t == 0 && valid && !LDS_IN) lds[0] = load(0);
         __syncthreads();
       } else if (t == 0 && valid) {
-        store(0, load(0));
+        store(0, LDS_IN ? lds[0] : load(0));
       }
       return;
     }
     for (int s = 0; s < ns; ++s) {
+      const bool lds_in = LDS_IN && s == 0;
       switch (d->radix[s]) {
-#define PLF_DYN_CASE(R)                                                   \
-  case R:                                                                 \
-    stage<R, TO_LDS>(d, s, lds, tw, t, valid, load, store);               \
+#define PLF_DYN_CASE(R)                                                      \
+  case R:                                                                    \
+    stage<R, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);          \
     break;
         PLF_DYN_CASE(2)
         PLF_DYN_CASE(3)
@@ -213,13 +217,13 @@ struct DynamicEngine {
         PLF_DYN_CASE(16)
 #undef PLF_DYN_CASE
         case 17:
-          if constexpr (BIGP) stage<17, TO_LDS>(d, s, lds, tw, t, valid, load, store);
+          if constexpr (BIGP) stage<17, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
           break;
         case 19:
-          if constexpr (BIGP) stage<19, TO_LDS>(d, s, lds, tw, t, valid, load, store);
+          if constexpr (BIGP) stage<19, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
           break;
         case 23:
-          if constexpr (BIGP) stage<23, TO_LDS>(d, s, lds, tw, t, valid, load, store);
+          if constexpr (BIGP) stage<23, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
           break;
         default:
           break;

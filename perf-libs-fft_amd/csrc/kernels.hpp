@@ -27,10 +27,10 @@ struct SEng {
   using E = StaticEngine<C, TPT, PAD, Ch>;
   static constexpr int NFIX = Ch::N;
   static __host__ __device__ constexpr int lds_per(int64_t) { return lds_padded<PAD>(Ch::N); }
-  template <bool TO_LDS, class Load, class Store>
+  template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
   static __device__ __forceinline__ void run(const GenericDesc *, cpx<C> *lds, const cpx<C> *tw,
                                              int t, bool valid, Load &&load, Store &&store) {
-    E::template run<TO_LDS>(lds, tw, t, valid, load, store);
+    E::template run<TO_LDS, LDS_IN>(lds, tw, t, valid, load, store);
   }
 };
 
@@ -44,10 +44,10 @@ struct DEng {
   static constexpr bool BIGP = TPT_ <= 256 && TPB_ <= 16 && TPT_ * TPB_ <= 256;
   using E = DynamicEngine<C, TPT, PTS, BIGP>;
   static __host__ __device__ constexpr int lds_per(int64_t n) { return lds_padded<PAD>((int)n); }
-  template <bool TO_LDS, class Load, class Store>
+  template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
   static __device__ __forceinline__ void run(const GenericDesc *d, cpx<C> *lds, const cpx<C> *tw,
                                              int t, bool valid, Load &&load, Store &&store) {
-    E::template run<TO_LDS>(d, lds, tw, t, valid, load, store);
+    E::template run<TO_LDS, LDS_IN>(d, lds, tw, t, valid, load, store);
   }
 };
 

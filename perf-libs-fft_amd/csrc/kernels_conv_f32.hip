@@ -1,0 +1,27 @@
+// kernels_conv_f32.hip -- f32 Bluestein / Rader kernel configurations.
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_conv_f32(std::vector<KEntry> &r) {
+    using S = float;
+    constexpr int P = P_F32;
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,    8,  32, 8, 8);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,    8,  32, 16, 8);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,   16,  16, 16, 16);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,   32,   8, 16, 16, 2);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,   64,   4, 16, 16, 4);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,  128,   2, 16, 16, 8);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,  256,   1, 16, 16, 16);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,  512,   1, 16, 16, 16, 2);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P, 1024,   1, 16, 16, 16, 4);
+    PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P,  64,   4, 16);
+    PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 256,   1, 16);
+    PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 1024,  1, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P,   16,  16, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P,   64,   4, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P,  256,   1, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P, 1024,   1, 16);
+}
+
+}  // namespace plf

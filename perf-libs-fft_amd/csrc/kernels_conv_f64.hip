@@ -1,0 +1,25 @@
+// kernels_conv_f64.hip -- f64 Bluestein / Rader kernel configurations.
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_conv_f64(std::vector<KEntry> &r) {
+    using S = double;
+    constexpr int P = P_F64;
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,    8,  32, 8, 8);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,   16,  16, 8, 4, 4);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,   32,   8, 8, 8, 4);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,   64,   4, 8, 8, 8);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,  128,   2, 8, 8, 4, 4);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,  256,   1, 8, 8, 8, 4);
+    PLF_STATIC(k_bluestein, KK_BLUE, S, P,  512,   1, 8, 8, 8, 8);
+    PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P,  64,   4, 16);
+    PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 256,   1, 16);
+    PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 1024,  1, 8);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P,   16,  16, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P,   64,   4, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P,  256,   1, 16);
+    PLF_DYNAMIC(k_rader, KK_RADER, S, P, 1024,   1, 8);
+}
+
+}  // namespace plf

@@ -1,0 +1,35 @@
+// kernels_r2r_f32.hip -- DCT-II / DST-II and DCT-III / DST-III kernel configurations.
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_r2r_f32(std::vector<KEntry> &r) {
+  {
+    using S = float;
+    constexpr int P = P_F32;
+    PLF_STATIC(k_dct2_even, KK_DCT2_E, S, P, 16, 16, 16, 16);
+    PLF_STATIC(k_dct2_even, KK_DCT2_E, S, P, 128, 2, 16, 16, 8);
+    PLF_STATIC(k_dct2_even, KK_DCT2_E, S, P, 256, 1, 16, 16, 16);
+    PLF_STATIC(k_dct3_even, KK_DCT3_E, S, P, 16, 16, 16, 16);
+    PLF_STATIC(k_dct3_even, KK_DCT3_E, S, P, 128, 2, 16, 16, 8);
+    PLF_STATIC(k_dct3_even, KK_DCT3_E, S, P, 256, 1, 16, 16, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 16, 16, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 16, 16, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 16, 16, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 16, 16, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 1024, 1, 16);
+  }
+}
+
+}  // namespace plf

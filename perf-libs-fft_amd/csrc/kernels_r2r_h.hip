@@ -1,0 +1,41 @@
+// kernels_r2r_h.hip -- DCT-II / DST-II and DCT-III / DST-III kernel configurations.
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_r2r_h(std::vector<KEntry> &r) {
+  {
+    using S = half_pair;
+    constexpr int P = P_F16;
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 1024, 1, 16);
+  }
+  {
+    using S = bf16_pair;
+    constexpr int P = P_BF16;
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct2_even, KK_DCT2_E, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct3_even, KK_DCT3_E, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct2_full, KK_DCT2_F, S, P, 1024, 1, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 64, 4, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 256, 1, 16);
+    PLF_DYNAMIC(k_dct3_full, KK_DCT3_F, S, P, 1024, 1, 16);
+  }
+}
+
+}  // namespace plf

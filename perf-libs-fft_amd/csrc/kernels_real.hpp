@@ -58,7 +58,8 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even(KArgs a) {
       [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, [&](int, cpx<C>) {});
   if (!valid) return;
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
-  const C h = C(0.5);
+  // flags bit1: interior bins doubled (the adjoint of C2R)
+  const C h = (a.flags & 2) ? C(1) : C(0.5);
   for (int k = t; 2 * k <= m; k += TPT) {
     if (k == 0) {
       const cpx<C> z0 = lds[lds_pad<Eng::PAD>(0)];
@@ -97,6 +98,7 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(m);
   const int64_t is = a.istride, os = a.ostride;
   const bool pair_ok = a.flags & 1;
+  const C hw = (a.flags & 2) ? C(0.5) : C(1);  // adjoint of R2C: interior bins halved
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
   Eng::template run<false>(
       a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
@@ -106,6 +108,9 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
         if (e == 0) {
           A.y = C(0);
           B.y = C(0);
+        } else {
+          A = {hw * A.x, hw * A.y};
+          B = {hw * B.x, hw * B.y};
         }
         const cpx<C> s = {A.x + B.x, A.y - B.y};   // A + conj B
         const cpx<C> d = {A.x - B.x, A.y + B.y};   // A - conj B
@@ -147,7 +152,10 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_full(KArgs a) {
       a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
       [&](int e) { return cpx<C>{ST::ldr(x[(int64_t)e * is]), C(0)}; },
       [&](int e, cpx<C> v) {
-        if (2 * e <= n) out[(int64_t)e * os] = ST::st(v);
+        if (2 * e <= n) {
+          if ((a.flags & 2) && e > 0 && 2 * e != n) v = {C(2) * v.x, C(2) * v.y};
+          out[(int64_t)e * os] = ST::st(v);
+        }
       });
 }
 
@@ -176,14 +184,221 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_full(KArgs a) {
         cpx<C> v;
         if (2 * e <= n) {
           v = ST::ld(X[(int64_t)e * is]);
-          if (e == 0 || 2 * e == n) v.y = C(0);
         } else {
           v = ST::ld(X[(int64_t)(n - e) * is]);
           v.y = -v.y;
         }
+        if (e == 0 || 2 * e == n) v.y = C(0);
+        else if (a.flags & 2) v = {C(0.5) * v.x, C(0.5) * v.y};
         return cswap(v);
       },
       [&](int e, cpx<C> v) { y[(int64_t)e * os] = ST::str(v.y); });  // Re of swap(v)
+}
+
+// ---------------------------------------------------------------------------
+// DCT-II / DST-II (FFTW REDFT10 / RODFT10), following the reference's
+// Makhoul steps (src/r2r_buffer_helper.hpp:318-335 and :457-470 for the input
+// permutation, :688-701 and :830-857 for the post-processing):
+//   v[i] = x[2i] (i < ceil(n/2)),  v[i] = x[2(n-i)-1] (otherwise),
+//   V = R2C_n(v),  y_k = 2 Re(w_k V_k),  y_{n-k} = -2 Im(w_k V_k),
+//   w_k = exp(-i pi k / 2n).
+// DST-II is the same with the odd-sourced v[i] negated and the output
+// reversed (y_{n-1-k}). flags bit1 scales the output of internal index 0 by
+// 1/2 (the adjoint of DCT-III / DST-III).
+// Even n uses the packed half-length engine (m = n/2): aux0 = W_n^k
+// (k <= m/2), aux1 = w_k (k <= m). Odd n runs the full-length engine with
+// aux1 = w_k (k <= n/2).
+// ---------------------------------------------------------------------------
+template <typename ST>
+__device__ __forceinline__ typename ST::C dct2_src(const typename ST::Rs *x, int i, int n,
+                                                  int64_t is, bool dst) {
+  using C = typename ST::C;
+  if (2 * i < n) return ST::ldr(x[(int64_t)(2 * i) * is]);
+  const C v = ST::ldr(x[(int64_t)(2 * (n - i) - 1) * is]);
+  return dst ? -v : v;
+}
+
+template <typename S, bool DST, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_dct2_even(KArgs a) {
+  using ST = StorageT<S>;
+  using Rs = typename ST::Rs;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const int m = (int)a.n;
+  const int n = 2 * m;
+  const Rs *x = static_cast<const Rs *>(a.in) + (valid ? tr * a.idist : 0);
+  Rs *y = static_cast<Rs *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(m);
+  const int64_t is = a.istride, os = a.ostride;
+  Eng::template run<true>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        return cpx<C>{dct2_src<ST>(x, 2 * e, n, is, DST), dct2_src<ST>(x, 2 * e + 1, n, is, DST)};
+      },
+      [&](int, cpx<C>) {});
+  if (!valid) return;
+  const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
+  const cpx<C> *w = static_cast<const cpx<C> *>(a.aux1);
+  const C h = C(0.5), two = C(2);
+  const C s0 = (a.flags & 2) ? C(0.5) : C(1);
+  auto put = [&](int o, C v) { y[(int64_t)(DST ? n - 1 - o : o) * os] = ST::str(v); };
+  // y_k and y_{n-k} from V_k
+  auto emit = [&](int k, cpx<C> V) {
+    const cpx<C> r = cmul(w[k], V);
+    put(k, two * r.x);
+    put(n - k, -two * r.y);
+  };
+  for (int k = t; 2 * k <= m; k += TPT) {
+    if (k == 0) {
+      const cpx<C> z0 = lds[lds_pad<Eng::PAD>(0)];
+      put(0, s0 * two * (z0.x + z0.y));
+      const C vm = z0.x - z0.y;  // V_m is real
+      put(m, two * vm * w[m].x);
+      continue;
+    }
+    const cpx<C> A = lds[lds_pad<Eng::PAD>(k)];
+    const cpx<C> B = lds[lds_pad<Eng::PAD>(m - k)];
+    const cpx<C> xe = {h * (A.x + B.x), h * (A.y - B.y)};
+    const cpx<C> xo = {h * (A.y + B.y), h * (B.x - A.x)};
+    const cpx<C> wx = cmul(W[k], xo);
+    emit(k, cadd(xe, wx));
+    if (2 * k != m) emit(m - k, {xe.x - wx.x, wx.y - xe.y});
+  }
+}
+
+template <typename S, bool DST, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_dct2_full(KArgs a) {
+  using ST = StorageT<S>;
+  using Rs = typename ST::Rs;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const int n = (int)a.n;
+  const Rs *x = static_cast<const Rs *>(a.in) + (valid ? tr * a.idist : 0);
+  Rs *y = static_cast<Rs *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(n);
+  const int64_t is = a.istride, os = a.ostride;
+  Eng::template run<true>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) { return cpx<C>{dct2_src<ST>(x, e, n, is, DST), C(0)}; },
+      [&](int, cpx<C>) {});
+  if (!valid) return;
+  const cpx<C> *w = static_cast<const cpx<C> *>(a.aux1);
+  const C two = C(2);
+  const C s0 = (a.flags & 2) ? C(0.5) : C(1);
+  auto put = [&](int o, C v) { y[(int64_t)(DST ? n - 1 - o : o) * os] = ST::str(v); };
+  for (int k = t; 2 * k <= n; k += TPT) {
+    const cpx<C> r = cmul(w[k], lds[lds_pad<Eng::PAD>(k)]);
+    if (k == 0) {
+      put(0, s0 * two * r.x);
+    } else {
+      put(k, two * r.x);
+      if (2 * k != n) put(n - k, -two * r.y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// DCT-III / DST-III (REDFT01 / RODFT01): the inverse Makhoul steps,
+//   X_k = (u_k - i u_{n-k}) conj(w_k)  (u_n = 0),  v = C2R_n(X),
+//   y[2i] = v_i (i < ceil(n/2)),  y[2(n-i)-1] = v_i (otherwise).
+// DST-III(u)_j = (-1)^j DCT-III(reverse u)_j. flags bit1 doubles the input
+// of internal index 0 (the adjoint of DCT-II / DST-II).
+// Even n: packed half-length engine, aux0 = W_n^k (k < m), aux1 = w_k (k <= m).
+// Odd n: full-length engine, aux1 = w_k (k < n).
+// ---------------------------------------------------------------------------
+template <typename ST, bool DST>
+__device__ __forceinline__ typename ST::C dct3_in(const typename ST::Rs *x, int k, int n,
+                                                 int64_t is, typename ST::C s0) {
+  using C = typename ST::C;
+  if (k >= n) return C(0);
+  const int src = DST ? n - 1 - k : k;
+  C v = ST::ldr(x[(int64_t)src * is]);
+  return k == 0 ? s0 * v : v;
+}
+
+template <typename S, bool DST, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_dct3_even(KArgs a) {
+  using ST = StorageT<S>;
+  using Rs = typename ST::Rs;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const int m = (int)a.n;
+  const int n = 2 * m;
+  const Rs *x = static_cast<const Rs *>(a.in) + (valid ? tr * a.idist : 0);
+  Rs *y = static_cast<Rs *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(m);
+  const int64_t is = a.istride, os = a.ostride;
+  const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
+  const cpx<C> *w = static_cast<const cpx<C> *>(a.aux1);
+  const C s0 = (a.flags & 2) ? C(2) : C(1);
+  auto X = [&](int k) {  // (u_k - i u_{n-k}) * conj(w_k), k in [0, m]
+    const C ur = dct3_in<ST, DST>(x, k, n, is, s0);
+    const C ui = k == 0 ? C(0) : -dct3_in<ST, DST>(x, n - k, n, is, s0);
+    return cconj_mul(cpx<C>{ur, ui}, w[k]);
+  };
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        const cpx<C> A = X(e);
+        const cpx<C> B = X(m - e);
+        const cpx<C> sm = {A.x + B.x, A.y - B.y};
+        const cpx<C> d = {A.x - B.x, A.y + B.y};
+        const cpx<C> wd = cconj_mul(d, W[e]);
+        return cswap(cpx<C>{sm.x - wd.y, sm.y + wd.x});
+      },
+      [&](int e, cpx<C> v) {
+        v = cswap(v);
+        const int i0 = 2 * e, i1 = 2 * e + 1;
+        const int o0 = 2 * i0 < n ? 2 * i0 : 2 * (n - i0) - 1;
+        const int o1 = 2 * i1 < n ? 2 * i1 : 2 * (n - i1) - 1;
+        const C v0 = (DST && (o0 & 1)) ? -v.x : v.x;
+        const C v1 = (DST && (o1 & 1)) ? -v.y : v.y;
+        y[(int64_t)o0 * os] = ST::str(v0);
+        y[(int64_t)o1 * os] = ST::str(v1);
+      });
+}
+
+template <typename S, bool DST, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_dct3_full(KArgs a) {
+  using ST = StorageT<S>;
+  using Rs = typename ST::Rs;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const int n = (int)a.n;
+  const Rs *x = static_cast<const Rs *>(a.in) + (valid ? tr * a.idist : 0);
+  Rs *y = static_cast<Rs *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(n);
+  const int64_t is = a.istride, os = a.ostride;
+  const cpx<C> *w = static_cast<const cpx<C> *>(a.aux1);
+  const C s0 = (a.flags & 2) ? C(2) : C(1);
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int k) {
+        const C ur = dct3_in<ST, DST>(x, k, n, is, s0);
+        const C ui = k == 0 ? C(0) : -dct3_in<ST, DST>(x, n - k, n, is, s0);
+        return cswap(cconj_mul(cpx<C>{ur, ui}, w[k]));
+      },
+      [&](int i, cpx<C> v) {
+        const int o = 2 * i < n ? 2 * i : 2 * (n - i) - 1;
+        const C r = v.y;  // Re of swap(v)
+        y[(int64_t)o * os] = ST::str((DST && (o & 1)) ? -r : r);
+      });
 }
 
 }  // namespace plf

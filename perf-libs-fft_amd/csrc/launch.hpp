@@ -5,6 +5,7 @@
 
 #include "kernels.hpp"
 #include "kernels_real.hpp"
+#include "kernels_conv.hpp"
 #include "registry.hpp"
 
 namespace plf {

@@ -30,6 +30,9 @@ struct Problem {
   int64_t istride = 1, idist = 0, ostride = 1, odist = 0;
   bool may_alias = false;
   int rigor = 1;
+  // 1: plan the adjoint of the transform described above (input/output
+  // layouts already swapped by the caller)
+  int adjoint = 0;
 };
 
 // Owning device allocation.
@@ -65,6 +68,7 @@ struct PlanError {
 std::unique_ptr<Node> build_node(const Problem &p, std::string *why);
 std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why);
 std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why);
+std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why);
 
 // ---- planner internals shared by the node builders ----------------------
 // A kernel choice for one engine length: the registry entry plus the chain.

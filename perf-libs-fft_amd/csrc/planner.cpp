@@ -8,8 +8,8 @@
 //   2. four-step: N = n1*n2, column pass + twiddle into a workspace, row
 //      pass with a transposed store (two HBM round trips, batch chunked so
 //      the workspace stays small);
-//   3. Bluestein / Rader for lengths with larger prime factors
-//      (bluestein.cpp).
+//   3. Rader / Bluestein for lengths with larger prime factors
+//      (conv_plan.cpp).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -74,7 +74,10 @@ const std::vector<KEntry> &kernel_registry() {
     register_kernels_f32_real(r);
     register_kernels_f64(r);
     register_kernels_f64_real(r);
-    register_kernels_h_real(r);
+    register_kernels_f16_real(r);
+    register_kernels_bf16_real(r);
+    register_kernels_conv(r);
+    register_kernels_r2r(r);
     register_kernels_f16(r);
     register_kernels_bf16(r);
     return r;
@@ -375,16 +378,13 @@ std::unique_ptr<Node> build_node(const Problem &p, std::string *why) {
     case K_R2C:
     case K_C2R:
       return build_real_node(p, why);
+    case K_R2R:
+      return build_r2r_node(p, why);
     default:
       *why = "transform kind not implemented on the GPU yet";
       return nullptr;
   }
 }
 
-// Placeholder until the chirp-z path lands.
-std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
-  *why = "no GPU kernel for C2C n=" + std::to_string(p.n) + " " + prec_name(p.prec);
-  return nullptr;
-}
 
 }  // namespace plf

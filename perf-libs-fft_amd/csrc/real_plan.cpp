@@ -2,6 +2,7 @@
 //
 // Even n runs the half-length packed kernels; odd n (or an even n whose
 // half length has no kernel) runs the full-length real-input kernels.
+#include <algorithm>
 #include <cstdint>
 #include <sstream>
 
@@ -23,9 +24,9 @@ std::vector<double> root_table(int64_t n, int64_t count) {
 class RealNode : public Node {
  public:
   RealNode(bool r2c, bool half, Choice c, int64_t n, int prec, int64_t batch, int64_t is,
-           int64_t id, int64_t os, int64_t od)
+           int64_t id, int64_t os, int64_t od, int extra_flags)
       : r2c_(r2c), half_(half), c_(std::move(c)), n_(n), prec_(prec), batch_(batch), is_(is),
-        id_(id), os_(os), od_(od) {}
+        id_(id), os_(os), od_(od), extra_(extra_flags) {}
   int64_t len() const { return half_ ? n_ / 2 : n_; }
   hipError_t init() {
     hipError_t err = eng_.init(c_, len());
@@ -56,6 +57,7 @@ class RealNode : public Node {
                       ((uintptr_t)rp % (2 * rb)) == 0;
       a.flags = ok ? 1 : 0;
     }
+    a.flags |= extra_;
     const int64_t grid = (batch_ + c_.e->tpb - 1) / c_.e->tpb;
     return c_.e->launch(a, grid, c_.e->lds_bytes(len()), s);
   }
@@ -73,8 +75,81 @@ class RealNode : public Node {
   int64_t n_;
   int prec_;
   int64_t batch_, is_, id_, os_, od_;
+  int extra_;
   EngineData eng_;
   DevMem aux_;
+};
+
+// Real transform through a complex plan of the same length (lengths with no
+// real kernel: large primes, or n beyond one LDS tile). R2C: widen to
+// (x, 0), forward C2C, keep bins 0..n/2. C2R: Hermitian extension (swapped),
+// forward C2C (= swapped inverse), real part.
+class RealViaComplexNode : public Node {
+ public:
+  RealViaComplexNode(bool r2c, int64_t n, int prec, int64_t batch, int64_t is, int64_t id,
+                     int64_t os, int64_t od, int64_t chunk)
+      : r2c_(r2c), n_(n), prec_(prec), batch_(batch), is_(is), id_(id), os_(os), od_(od),
+        chunk_(chunk) {}
+  hipError_t init(std::string *why) {
+    Problem q;
+    q.kind = K_C2C;
+    q.dir = -1;
+    q.prec = prec_compute_bytes(prec_) == 8 ? P_F64 : P_F32;
+    q.n = n_;
+    q.howmany = chunk_;
+    q.istride = q.ostride = 1;
+    q.idist = q.odist = n_;
+    q.may_alias = true;
+    inner_ = build_c2c_node(q, why);
+    return inner_ ? hipSuccess : hipErrorInvalidValue;
+  }
+  size_t buf_bytes() const { return (size_t)chunk_ * (size_t)n_ * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return buf_bytes() + inner_->workspace_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    char *buf = static_cast<char *>(work);
+    void *iws = buf + buf_bytes();
+    const size_t rb = (size_t)prec_real_bytes(prec_);
+    const size_t ib = r2c_ ? rb : 2 * rb, ob = r2c_ ? 2 * rb : rb;
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk_) {
+      const int64_t cb = std::min(chunk_, batch_ - b0);
+      KArgs a{};
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * ib;
+      a.out = buf;
+      a.count = cb;
+      a.istride = is_;
+      a.idist = id_;
+      a.n = n_;
+      a.flags = flags_;
+      hipError_t err = launch_real_stage(r2c_ ? 0 : 2, prec_, a, cb * n_, s);
+      if (err == hipSuccess) err = inner_->enqueue(buf, buf, iws, s);
+      if (err != hipSuccess) return err;
+      KArgs c{};
+      c.in = buf;
+      c.out = static_cast<char *>(out) + (size_t)(b0 * od_) * ob;
+      c.count = cb;
+      c.ostride = os_;
+      c.odist = od_;
+      c.n = n_;
+      c.flags = flags_;
+      err = launch_real_stage(r2c_ ? 1 : 3, prec_, c, cb * (r2c_ ? n_ / 2 + 1 : n_), s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(" << (r2c_ ? "r2c" : "c2r") << "_via_c2c n=" << n_ << " chunk=" << chunk_
+      << " batch=" << batch_ << " " << inner_->describe() << ")";
+    return o.str();
+  }
+  int flags_ = 0;
+
+ private:
+  bool r2c_;
+  int64_t n_;
+  int prec_;
+  int64_t batch_, is_, id_, os_, od_, chunk_;
+  std::unique_ptr<Node> inner_;
 };
 
 std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why) {
@@ -89,12 +164,104 @@ std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why) {
   }
   if (!c.e) c = choose(r2c ? KK_R2C_F : KK_C2R_F, p.n, p.prec, false);
   if (!c.e) {
-    *why = std::string("no GPU kernel for ") + (r2c ? "R2C" : "C2R") + " n=" +
-           std::to_string(p.n) + " " + prec_name(p.prec);
-    return nullptr;
+    const int64_t per = p.n * 2 * prec_compute_bytes(p.prec);
+    const int64_t chunk =
+        std::max<int64_t>(1, std::min<int64_t>(p.howmany, (int64_t(64) << 20) / per));
+    auto node = std::make_unique<RealViaComplexNode>(r2c, p.n, p.prec, p.howmany, p.istride, id,
+                                                     p.ostride, od, chunk);
+    node->flags_ = p.adjoint ? 2 : 0;
+    if (node->init(why) != hipSuccess) return nullptr;
+    return node;
   }
   auto node = std::make_unique<RealNode>(r2c, half, c, p.n, p.prec, p.howmany, p.istride, id,
-                                         p.ostride, od);
+                                         p.ostride, od, p.adjoint ? 2 : 0);
+  hipError_t err = node->init();
+  if (err != hipSuccess) {
+    *why = std::string("device upload failed: ") + hipGetErrorString(err);
+    return nullptr;
+  }
+  return node;
+}
+
+// ---------------------------------------------------------------------------
+// DCT-II / DST-II / DCT-III / DST-III (r2r kinds 1, 5, 2, 6)
+// ---------------------------------------------------------------------------
+class DctNode : public Node {
+ public:
+  DctNode(bool two, bool dst, bool half, Choice c, int64_t n, int prec, int64_t batch,
+          int64_t is, int64_t id, int64_t os, int64_t od, int flags)
+      : two_(two), dst_(dst), half_(half), c_(std::move(c)), n_(n), prec_(prec), batch_(batch),
+        is_(is), id_(id), os_(os), od_(od), flags_(flags) {}
+  int64_t len() const { return half_ ? n_ / 2 : n_; }
+  hipError_t init() {
+    hipError_t err = eng_.init(c_, len());
+    if (err != hipSuccess) return err;
+    if (half_) err = upload_table(aux0_, root_table(n_, n_ / 2 + 1), prec_);
+    if (err != hipSuccess) return err;
+    // w_k = exp(-i pi k / 2n) = W_{4n}^k
+    return upload_table(aux1_, root_table(4 * n_, half_ ? n_ / 2 + 1 : n_), prec_);
+  }
+  hipError_t enqueue(const void *in, void *out, void *, hipStream_t s) const override {
+    if (batch_ <= 0) return hipSuccess;
+    KArgs a{};
+    a.in = in;
+    a.out = out;
+    a.tw = eng_.tw.p;
+    a.desc = eng_.d();
+    a.aux0 = aux0_.p;
+    a.aux1 = aux1_.p;
+    a.count = batch_;
+    a.istride = is_;
+    a.idist = id_;
+    a.ostride = os_;
+    a.odist = od_;
+    a.n = len();
+    a.flags = flags_;
+    const int64_t grid = (batch_ + c_.e->tpb - 1) / c_.e->tpb;
+    return c_.e->launch(a, grid, c_.e->lds_bytes(len()), s);
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(" << (dst_ ? "dst" : "dct") << (two_ ? "2" : "3") << (flags_ & 2 ? "_adjoint" : "")
+      << " n=" << n_ << " " << chain_str(c_) << " batch=" << batch_ << " is=" << is_
+      << " id=" << id_ << " os=" << os_ << " od=" << od_ << ")";
+    return o.str();
+  }
+
+ private:
+  bool two_, dst_, half_;
+  Choice c_;
+  int64_t n_;
+  int prec_;
+  int64_t batch_, is_, id_, os_, od_;
+  int flags_;
+  EngineData eng_;
+  DevMem aux0_, aux1_;
+};
+
+std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why) {
+  // plfft_r2r_kind_t values: DCT_2 = 1, DCT_3 = 2, DST_2 = 5, DST_3 = 6
+  const int k = p.r2r_kind;
+  if (k != 1 && k != 2 && k != 5 && k != 6) {
+    *why = "r2r kind " + std::to_string(k) + " has no GPU implementation yet";
+    return nullptr;
+  }
+  const bool two = (k == 1 || k == 5), dst = (k == 5 || k == 6);
+  const int64_t id = p.howmany > 1 ? p.idist : 0;
+  const int64_t od = p.howmany > 1 ? p.odist : 0;
+  Choice c;
+  bool half = false;
+  if (p.n % 2 == 0) {
+    c = choose(two ? KK_DCT2_E : KK_DCT3_E, p.n / 2, p.prec, dst);
+    half = c.e != nullptr;
+  }
+  if (!c.e) c = choose(two ? KK_DCT2_F : KK_DCT3_F, p.n, p.prec, dst);
+  if (!c.e) {
+    *why = "no GPU kernel for r2r kind " + std::to_string(k) + " n=" + std::to_string(p.n);
+    return nullptr;
+  }
+  auto node = std::make_unique<DctNode>(two, dst, half, c, p.n, p.prec, p.howmany, p.istride, id,
+                                        p.ostride, od, p.adjoint ? 2 : 0);
   hipError_t err = node->init();
   if (err != hipSuccess) {
     *why = std::string("device upload failed: ") + hipGetErrorString(err);

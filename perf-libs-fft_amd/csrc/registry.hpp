@@ -28,10 +28,18 @@ enum KKind : int {
   KK_C2R_E = 4,  // C2R, even n, engine length n/2
   KK_R2C_F = 5,  // R2C, full-length engine
   KK_C2R_F = 6,  // C2R, full-length engine
+  KK_BLUE = 7,   // fused Bluestein, engine length M >= 2n-1
+  KK_RADER = 8,  // fused Rader, engine length n-1
+  KK_DCT2_E = 9,   // DCT-II (bwd slot: DST-II), even n, engine length n/2
+  KK_DCT2_F = 10,  // DCT-II / DST-II, full-length engine
+  KK_DCT3_E = 11,  // DCT-III (bwd slot: DST-III), even n, engine length n/2
+  KK_DCT3_F = 12,  // DCT-III / DST-III, full-length engine
 };
 inline const char *kkind_name(int k) {
-  static const char *names[] = {"c2c", "col", "row", "r2c_half", "c2r_half", "r2c_full", "c2r_full"};
-  return (k >= 0 && k <= 6) ? names[k] : "?";
+  static const char *names[] = {"c2c",      "col",      "row",       "r2c_half", "c2r_half",
+                                "r2c_full", "c2r_full", "bluestein", "rader",    "dct2_half",
+                                "dct2_full", "dct3_half", "dct3_full"};
+  return (k >= 0 && k <= 12) ? names[k] : "?";
 }
 
 using KLaunch = hipError_t (*)(const KArgs &, int64_t grid, size_t lds, hipStream_t);
@@ -65,7 +73,23 @@ void register_kernels_f32(std::vector<KEntry> &r);
 void register_kernels_f32_4s(std::vector<KEntry> &r);
 void register_kernels_f32_real(std::vector<KEntry> &r);
 void register_kernels_f64_real(std::vector<KEntry> &r);
-void register_kernels_h_real(std::vector<KEntry> &r);
+void register_kernels_conv(std::vector<KEntry> &r);
+void register_kernels_r2r(std::vector<KEntry> &r);
+void register_kernels_conv_f32(std::vector<KEntry> &r);
+void register_kernels_conv_f64(std::vector<KEntry> &r);
+void register_kernels_conv_f16(std::vector<KEntry> &r);
+void register_kernels_conv_bf16(std::vector<KEntry> &r);
+void register_kernels_f16_real(std::vector<KEntry> &r);
+void register_kernels_bf16_real(std::vector<KEntry> &r);
+
+// Element-wise chirp-z helpers for lengths beyond one LDS tile
+// (kernels_conv.hip); `prec` selects the storage type.
+hipError_t launch_chirp_in(int prec, bool bwd, const KArgs &a, hipStream_t s);
+hipError_t launch_pointwise_swap(int prec, const KArgs &a, hipStream_t s);
+hipError_t launch_chirp_out(int prec, bool bwd, const KArgs &a, hipStream_t s);
+// Real <-> complex staging (kernels_conv.hip): which = 0 widen, 1 narrow,
+// 2 hermitian, 3 realpart; `total` threads.
+hipError_t launch_real_stage(int which, int prec, const KArgs &a, int64_t total, hipStream_t s);
 void register_kernels_f64(std::vector<KEntry> &r);
 void register_kernels_f16(std::vector<KEntry> &r);
 void register_kernels_bf16(std::vector<KEntry> &r);

@@ -219,3 +219,65 @@ def test_four_step_strided_batch(torch_cuda, orc):
     plan.execute(xd, yd)
     ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, hm, 1, 1, n, n * hm)
     assert relerr(from_dev(torch, yd), ref) < TOL["f32"]
+
+
+# Lengths with prime factors above 23: fused Rader (n prime, n-1 smooth),
+# fused Bluestein, and chirp-z around the four-step transform; includes the
+# prime configuration's 97, 1009 (Rader), 65537 and the reference's
+# alias-test primes 509, 2053, 8191 (test/plfft_may_alias_test.cpp:40-52).
+PRIMES = [29, 31, 37, 47, 53, 97, 101, 509, 1009, 2053, 4099, 8191, 2 * 1009, 29 * 31]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", PRIMES)
+def test_c2c_prime_lengths(torch_cuda, orc, prec, n):
+    torch = torch_cuda
+    hm = max(1, min(12, (1 << 16) // n)) + 1
+    x = rng.complex_normal(n + 3, n * hm).astype(CPLX_NP[prec])
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    for direction in (-1, 1):
+        plan = P.Plan("c2c", n, direction, prec, howmany=hm)
+        plan.execute(xd, yd)
+        ref = orc.c2c_batched(x.astype(np.complex128), n, direction, hm, 1, n, 1, n, n * hm)
+        assert relerr(from_dev(torch, yd), ref) < TOL[prec], plan.describe()
+
+
+def test_c2c_prime_plan_choice():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    assert "rader" in P.Plan("c2c", 97, -1, "f32").describe()
+    assert "rader" in P.Plan("c2c", 1009, -1, "f32").describe()
+    # 47 - 1 = 2*23 has an engine here (Rader); 59 - 1 = 2*29 does not
+    assert "rader" in P.Plan("c2c", 47, -1, "f32").describe()
+    assert "bluestein" in P.Plan("c2c", 59, -1, "f32").describe()
+
+
+@pytest.mark.parametrize("n,prec", [(65537, "f32"), (65537, "f64"), (40009, "f32")])
+def test_c2c_large_prime_chirp_z(torch_cuda, orc, n, prec):
+    torch = torch_cuda
+    hm = 3
+    x = rng.complex_normal(n, n * hm).astype(CPLX_NP[prec])
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    plan = P.Plan("c2c", n, -1, prec, howmany=hm)
+    plan.execute(xd, yd)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
+    assert relerr(from_dev(torch, yd), ref) < TOL[prec] * 2, plan.describe()
+
+
+@pytest.mark.parametrize("n", [97, 509, 2053, 8191])
+def test_prime_known_pairs(torch_cuda, n):
+    """delta -> constant and pure tone at k0 = n//3 (prime configuration)."""
+    torch = torch_cuda
+    x = np.zeros((2, n), np.complex64)
+    x[0, 0] = 1
+    k0 = n // 3
+    x[1, k0] = 1
+    xd = to_dev(torch, x)
+    yd = torch.empty_like(xd)
+    P.Plan("c2c", n, -1, "f32", howmany=2).execute(xd, yd)
+    y = from_dev(torch, yd)
+    assert np.max(np.abs(y[0] - 1)) < 1e-5
+    assert np.max(np.abs(y[1] - np.exp(-2j * np.pi * np.arange(n) * k0 / n))) < 1e-5

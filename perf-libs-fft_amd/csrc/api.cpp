@@ -297,7 +297,9 @@ extern "C" plfft_status_t plfft_plan_create_from_config(plfft_plan_t **plan, con
     return PLFFT_PLAN_CREATION_FAILED;
   }
   std::string why;
+  plf::set_planning_rigor(p.rigor);
   std::unique_ptr<plf::Node> node = plf::build_node(p, &why);
+  plf::set_planning_rigor(0);
   if (!node) {
     plf::set_last_error(why);
     return PLFFT_PLAN_CREATION_FAILED;

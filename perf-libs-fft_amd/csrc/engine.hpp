@@ -30,7 +30,10 @@ namespace plf {
 // ---------------------------------------------------------------------------
 // compile-time radix chain
 // ---------------------------------------------------------------------------
-template <typename C, int TPT, int PAD, typename Ch>
+// PAIR: thread t owns the KB consecutive butterflies t*KB .. t*KB+KB-1 of a
+// stage (instead of t, t+TPT, ...), so that with unit stride its global
+// loads and stores of neighbouring elements merge into wider accesses.
+template <typename C, int TPT, int PAD, typename Ch, bool PAIR = false>
 struct StaticEngine {
   static constexpr int N = Ch::N;
   static constexpr int NS = Ch::count;
@@ -47,9 +50,10 @@ struct StaticEngine {
     constexpr bool FIRST = (SI == 0);
     constexpr bool LAST = (SI == NS - 1);
     cpx<C> v[KB][R];
+    auto bidx = [&](int k) { return PAIR ? t * KB + k : t + k * TPT; };
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bidx(k);
       const bool act = (NB % TPT == 0) || (b < NB);
 #pragma unroll
       for (int m = 0; m < R; ++m) {
@@ -62,7 +66,7 @@ struct StaticEngine {
     }
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bidx(k);
       if constexpr (!FIRST) {
         const int j = b % L;
         const bool act = (NB % TPT == 0) || (b < NB);
@@ -77,7 +81,7 @@ struct StaticEngine {
     if constexpr ((!FIRST || LDS_IN) && (!LAST || TO_LDS)) __syncthreads();  // all reads done
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bidx(k);
       const bool act = (NB % TPT == 0) || (b < NB);
       if constexpr (LAST && !TO_LDS) {
         if (act && valid) {

@@ -316,8 +316,18 @@ struct StorageT<float> {
   using E = cpx<float>;
   using Rs = float;
   using C = float;
+  typedef float V2 __attribute__((ext_vector_type(2)));
   static __device__ __forceinline__ cpx<float> ld(const E &e) { return e; }
   static __device__ __forceinline__ E st(cpx<float> v) { return v; }
+  // streaming (non-temporal) element access for data touched exactly once
+  static __device__ __forceinline__ cpx<float> ldnt(const E *p) {
+    V2 v = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(p));
+    return {v.x, v.y};
+  }
+  static __device__ __forceinline__ void stnt(E *p, cpx<float> c) {
+    V2 v = {c.x, c.y};
+    __builtin_nontemporal_store(v, reinterpret_cast<V2 *>(p));
+  }
   static __device__ __forceinline__ float ldr(Rs r) { return r; }
   static __device__ __forceinline__ Rs str(float v) { return v; }
 };
@@ -326,8 +336,17 @@ struct StorageT<double> {
   using E = cpx<double>;
   using Rs = double;
   using C = double;
+  typedef double V2 __attribute__((ext_vector_type(2)));
   static __device__ __forceinline__ cpx<double> ld(const E &e) { return e; }
   static __device__ __forceinline__ E st(cpx<double> v) { return v; }
+  static __device__ __forceinline__ cpx<double> ldnt(const E *p) {
+    V2 v = __builtin_nontemporal_load(reinterpret_cast<const V2 *>(p));
+    return {v.x, v.y};
+  }
+  static __device__ __forceinline__ void stnt(E *p, cpx<double> c) {
+    V2 v = {c.x, c.y};
+    __builtin_nontemporal_store(v, reinterpret_cast<V2 *>(p));
+  }
   static __device__ __forceinline__ double ldr(Rs r) { return r; }
   static __device__ __forceinline__ Rs str(double v) { return v; }
 };
@@ -336,6 +355,14 @@ struct StorageT<half_pair> {
   using E = half_pair;
   using Rs = uint16_t;
   using C = float;
+  static __device__ __forceinline__ cpx<float> ldnt(const E *p) {
+    const uint32_t u = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+    return {h2f(uint16_t(u & 0xffffu)), h2f(uint16_t(u >> 16))};
+  }
+  static __device__ __forceinline__ void stnt(E *p, cpx<float> c) {
+    const uint32_t u = uint32_t(f2h(c.x)) | (uint32_t(f2h(c.y)) << 16);
+    __builtin_nontemporal_store(u, reinterpret_cast<uint32_t *>(p));
+  }
   static __device__ __forceinline__ cpx<float> ld(const E &e) {
     return {h2f(e.x), h2f(e.y)};
   }
@@ -348,6 +375,14 @@ struct StorageT<bf16_pair> {
   using E = bf16_pair;
   using Rs = uint16_t;
   using C = float;
+  static __device__ __forceinline__ cpx<float> ldnt(const E *p) {
+    const uint32_t u = __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(p));
+    return {bf2f(uint16_t(u & 0xffffu)), bf2f(uint16_t(u >> 16))};
+  }
+  static __device__ __forceinline__ void stnt(E *p, cpx<float> c) {
+    const uint32_t u = uint32_t(f2bf(c.x)) | (uint32_t(f2bf(c.y)) << 16);
+    __builtin_nontemporal_store(u, reinterpret_cast<uint32_t *>(p));
+  }
   static __device__ __forceinline__ cpx<float> ld(const E &e) {
     return {bf2f(e.x), bf2f(e.y)};
   }

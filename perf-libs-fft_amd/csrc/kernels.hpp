@@ -18,13 +18,13 @@
 
 namespace plf {
 
-template <typename C_, int TPT_, int... Rs>
-struct SEng {
+template <typename C_, int TPT_, int PAD_, bool PAIR_, int... Rs>
+struct SEngX {
   using C = C_;
   static constexpr int TPT = TPT_;
-  static constexpr int PAD = 5;
+  static constexpr int PAD = PAD_;
   using Ch = Chain<Rs...>;
-  using E = StaticEngine<C, TPT, PAD, Ch>;
+  using E = StaticEngine<C, TPT, PAD, Ch, PAIR_>;
   static constexpr int NFIX = Ch::N;
   static __host__ __device__ constexpr int lds_per(int64_t) { return lds_padded<PAD>(Ch::N); }
   template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
@@ -33,6 +33,8 @@ struct SEng {
     E::template run<TO_LDS, LDS_IN>(lds, tw, t, valid, load, store);
   }
 };
+template <typename C_, int TPT_, int... Rs>
+using SEng = SEngX<C_, TPT_, 5, false, Rs...>;
 
 template <typename C_, int TPT_, int PTS_, int TPB_>
 struct DEng {
@@ -81,6 +83,31 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2c(KArgs a) {
         return BWD ? cswap(v) : v;
       },
       [&](int e, cpx<C> v) { out[(int64_t)e * os] = ST::st(BWD ? cswap(v) : v); });
+}
+
+// batched C2C, unit element stride: streaming (non-temporal) global access;
+// with a PAIR engine neighbouring elements of one thread merge into 16-byte
+// accesses.
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_c2c_unit(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const E *in = static_cast<const E *>(a.in) + (valid ? tr * a.idist : 0);
+  E *out = static_cast<E *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        cpx<C> v = ST::ldnt(in + e);
+        return BWD ? cswap(v) : v;
+      },
+      [&](int e, cpx<C> v) { ST::stnt(out + e, BWD ? cswap(v) : v); });
 }
 
 // W_N^e from the split table: hi[e >> shift] * lo[e & (2^shift - 1)].

@@ -1,4 +1,5 @@
 // kernels_bf16_real.hip -- bf16 storage real-data (R2C / C2R) kernels.
+#include "kernels_real.hpp"
 #include "launch.hpp"
 
 namespace plf {

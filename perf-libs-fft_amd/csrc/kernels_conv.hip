@@ -1,4 +1,5 @@
 // kernels_conv.hip -- element-wise chirp-z helpers for long lengths.
+#include "kernels_elem.hpp"
 #include "launch.hpp"
 
 namespace plf {
@@ -68,7 +69,9 @@ static hipError_t real_stage_t(int which, const KArgs &a, int64_t total, hipStre
     case 0: hipLaunchKernelGGL((k_widen<S>), grid, dim3(256), 0, s, a); break;
     case 1: hipLaunchKernelGGL((k_narrow<S>), grid, dim3(256), 0, s, a); break;
     case 2: hipLaunchKernelGGL((k_hermitian<S>), grid, dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_realpart<S>), grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_realpart<S>), grid, dim3(256), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_dct_pre<S>), grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_dct_post<S>), grid, dim3(256), 0, s, a); break;
   }
   return hipGetLastError();
 }

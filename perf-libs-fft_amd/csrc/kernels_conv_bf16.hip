@@ -1,4 +1,5 @@
 // kernels_conv_bf16.hip -- bf16 Bluestein / Rader kernel configurations.
+#include "kernels_conv.hpp"
 #include "launch.hpp"
 
 namespace plf {

@@ -1,4 +1,5 @@
 // kernels_conv_f32.hip -- f32 Bluestein / Rader kernel configurations.
+#include "kernels_conv.hpp"
 #include "launch.hpp"
 
 namespace plf {

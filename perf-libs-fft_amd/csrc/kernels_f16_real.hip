@@ -1,4 +1,5 @@
 // kernels_f16_real.hip -- f16 storage real-data (R2C / C2R) kernels.
+#include "kernels_real.hpp"
 #include "launch.hpp"
 
 namespace plf {

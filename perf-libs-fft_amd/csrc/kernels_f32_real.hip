@@ -1,4 +1,5 @@
 // kernels_f32_real.hip -- FP32 real-data (R2C / C2R) kernel configurations.
+#include "kernels_real.hpp"
 #include "launch.hpp"
 
 namespace plf {

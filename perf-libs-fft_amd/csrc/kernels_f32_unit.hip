@@ -1,0 +1,25 @@
+// kernels_f32_unit.hip -- FP32 unit-stride streaming C2C configurations
+// (first entry per length is the default; the autotuner may pick another).
+#include "launch.hpp"
+
+namespace plf {
+
+void register_kernels_f32_unit(std::vector<KEntry> &r) {
+  using S = float;
+  constexpr int P = P_F32;
+  //              TPT  TPB PAD PAIR   radix chain
+  PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
+  PLF_UNIT(S, P,   8,  32, 4, true,  16, 16);
+  PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
+  PLF_UNIT(S, P,  32,   8, 4, true,  16, 16, 4);
+  PLF_UNIT(S, P, 128,   2, 4, true,  8, 8, 8, 4);
+  PLF_UNIT(S, P, 256,   1, 4, false, 16, 16, 16);
+  PLF_UNIT(S, P, 256,   1, 5, false, 16, 16, 16);
+  PLF_UNIT(S, P, 128,   2, 4, true,  16, 16, 16);
+  PLF_UNIT(S, P, 256,   1, 4, true,  8, 8, 8, 8);
+  PLF_UNIT(S, P, 256,   2, 4, true,  8, 8, 8, 8);
+  PLF_UNIT(S, P, 1024,  1, 4, false, 16, 16, 16, 4);
+  PLF_UNIT(S, P, 512,   1, 4, true,  16, 16, 16, 4);
+}
+
+}  // namespace plf

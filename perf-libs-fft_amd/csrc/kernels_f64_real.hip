@@ -1,4 +1,5 @@
 // kernels_f64_real.hip -- FP64 real-data (R2C / C2R) kernel configurations.
+#include "kernels_real.hpp"
 #include "launch.hpp"
 
 namespace plf {

@@ -1,4 +1,5 @@
 // kernels_r2r_h.hip -- DCT-II / DST-II and DCT-III / DST-III kernel configurations.
+#include "kernels_real.hpp"
 #include "launch.hpp"
 
 namespace plf {

@@ -4,8 +4,6 @@
 #include <initializer_list>
 
 #include "kernels.hpp"
-#include "kernels_real.hpp"
-#include "kernels_conv.hpp"
 #include "registry.hpp"
 
 namespace plf {
@@ -101,3 +99,17 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
       &::plf::launch_k<                                                                    \
           &::plf::KERNEL<S, false, TPB, ::plf::DEng<typename ::plf::StorageT<S>::C, TPT, PTS, TPB>>, \
           TPT * TPB>)
+
+// Unit-stride streaming C2C with an explicit LDS padding shift and
+// butterfly-pairing choice.
+#define PLF_UNIT(S, PREC, TPT, TPB, PAD, PAIR, ...)                                       \
+  do {                                                                                    \
+    using EngT = ::plf::SEngX<typename ::plf::StorageT<S>::C, TPT, PAD, PAIR, __VA_ARGS__>; \
+    ::plf::add_entry(r, ::plf::KK_C2C_UNIT, PREC, ::plf::Chain<__VA_ARGS__>::N,              \
+                     {__VA_ARGS__}, TPT, TPB, PAIR ? 1 : 0,                               \
+                     ::plf::Chain<__VA_ARGS__>::count > 1,                                \
+                     &::plf::launch_k<&::plf::k_c2c_unit<S, false, TPB, EngT>, TPT * TPB>, \
+                     &::plf::launch_k<&::plf::k_c2c_unit<S, true, TPB, EngT>, TPT * TPB>); \
+    r[r.size() - 1].pad = PAD;                                                            \
+    r[r.size() - 2].pad = PAD;                                                            \
+  } while (0)

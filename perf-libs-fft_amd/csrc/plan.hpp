@@ -92,6 +92,14 @@ struct EngineData {
   const GenericDesc *d() const { return static_cast<const GenericDesc *>(desc.p); }
 };
 
+// Autotuner (autotune.cpp): index of the candidate to use among entries for
+// one (kind, n, precision, direction); times them under MEASURE or above,
+// caches the winner in memory and in the wisdom file. -1 when the
+// PLFFT_KERNEL_FILTER diagnostic excludes every candidate.
+int tune_pick(int kind, int64_t n, int prec, bool bwd, const std::vector<Choice> &cands);
+void set_planning_rigor(int rigor);
+int planning_rigor();
+
 // W_n^k (forward sign) for k in [0, count), interleaved re/im, FP64.
 std::vector<double> root_table(int64_t n, int64_t count);
 

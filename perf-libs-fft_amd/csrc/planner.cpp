@@ -52,6 +52,7 @@ std::string KEntry::name() const {
     o << "_dyn";
   }
   o << "_t" << tpt << "_b" << tpb;
+  if (kind == KK_C2C_UNIT) o << "_p" << pad << (pts ? "_pair" : "");
   return o.str();
 }
 
@@ -71,6 +72,7 @@ const std::vector<KEntry> &kernel_registry() {
     std::vector<KEntry> r;
     register_kernels_f32(r);
     register_kernels_f32_4s(r);
+    register_kernels_f32_unit(r);
     register_kernels_f32_real(r);
     register_kernels_f64(r);
     register_kernels_f64_real(r);
@@ -112,13 +114,20 @@ std::vector<int> dynamic_chain(int64_t n) {
 // entry with the fewest threads that fits.
 Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div) {
   Choice c;
+  std::vector<Choice> stat;
   for (const auto &e : kernel_registry()) {
     if (e.kind != kind || e.prec != prec || e.bwd != bwd || e.n != len) continue;
     if (need_tpb_div && need_tpb_div % e.tpb) continue;
     if (e.lds_bytes(len) > 160 * 1024) continue;
-    c.e = &e;
-    c.chain = e.chain;
-    return c;
+    Choice s;
+    s.e = &e;
+    s.chain = e.chain;
+    stat.push_back(s);
+  }
+  if (!stat.empty()) {
+    const int pick = tune_pick(kind, len, prec, bwd, stat);
+    if (pick >= 0) return stat[pick];
+    if (kind == KK_C2C_UNIT) return c;  // filtered out: the planner falls back
   }
   std::vector<int> ch = len == 1 ? std::vector<int>{} : dynamic_chain(len);
   if (len != 1 && ch.empty()) return c;
@@ -324,8 +333,10 @@ std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why) {
   const bool bwd = p.dir > 0;
   const int64_t id = p.howmany > 1 ? p.idist : 0;
   const int64_t od = p.howmany > 1 ? p.odist : 0;
-  // 1. one fused kernel
-  Choice c = choose(KK_C2C, p.n, p.prec, bwd);
+  // 1. one fused kernel (the streaming unit-stride variant when it applies)
+  Choice c;
+  if (p.istride == 1 && p.ostride == 1) c = choose(KK_C2C_UNIT, p.n, p.prec, bwd);
+  if (!c.e) c = choose(KK_C2C, p.n, p.prec, bwd);
   if (c.e) {
     auto node = std::make_unique<BatchC2CNode>(c, p.n, p.howmany, p.istride, id, p.ostride, od);
     hipError_t err = node->init();

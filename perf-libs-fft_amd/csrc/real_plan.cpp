@@ -239,6 +239,74 @@ class DctNode : public Node {
   DevMem aux0_, aux1_;
 };
 
+// DCT/DST II/III through a C2C plan (any length): pre-processing kernel,
+// forward C2C in the compute precision, post-processing kernel.
+class DctViaComplexNode : public Node {
+ public:
+  DctViaComplexNode(int flags, int64_t n, int prec, int64_t batch, int64_t is, int64_t id,
+                    int64_t os, int64_t od, int64_t chunk)
+      : flags_(flags), n_(n), prec_(prec), batch_(batch), is_(is), id_(id), os_(os), od_(od),
+        chunk_(chunk) {}
+  hipError_t init(std::string *why) {
+    Problem q;
+    q.kind = K_C2C;
+    q.dir = -1;
+    q.prec = prec_compute_bytes(prec_) == 8 ? P_F64 : P_F32;
+    q.n = n_;
+    q.howmany = chunk_;
+    q.istride = q.ostride = 1;
+    q.idist = q.odist = n_;
+    q.may_alias = true;
+    inner_ = build_c2c_node(q, why);
+    if (!inner_) return hipErrorInvalidValue;
+    return upload_table(w_, root_table(4 * n_, n_), prec_);
+  }
+  size_t buf_bytes() const { return (size_t)chunk_ * (size_t)n_ * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return buf_bytes() + inner_->workspace_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    char *buf = static_cast<char *>(work);
+    void *iws = buf + buf_bytes();
+    const size_t rb = (size_t)prec_real_bytes(prec_);
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk_) {
+      const int64_t cb = std::min(chunk_, batch_ - b0);
+      KArgs a{};
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * rb;
+      a.out = buf;
+      a.aux1 = w_.p;
+      a.count = cb;
+      a.istride = is_;
+      a.idist = id_;
+      a.n = n_;
+      a.flags = flags_;
+      hipError_t err = launch_real_stage(4, prec_, a, cb * n_, s);
+      if (err == hipSuccess) err = inner_->enqueue(buf, buf, iws, s);
+      if (err != hipSuccess) return err;
+      a.in = buf;
+      a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * rb;
+      a.ostride = os_;
+      a.odist = od_;
+      err = launch_real_stage(5, prec_, a, cb * n_, s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(" << (flags_ & 4 ? "dst" : "dct") << (flags_ & 8 ? "3" : "2")
+      << (flags_ & 2 ? "_adjoint" : "") << "_via_c2c n=" << n_ << " chunk=" << chunk_
+      << " batch=" << batch_ << " " << inner_->describe() << ")";
+    return o.str();
+  }
+
+ private:
+  int flags_;
+  int64_t n_;
+  int prec_;
+  int64_t batch_, is_, id_, os_, od_, chunk_;
+  std::unique_ptr<Node> inner_;
+  DevMem w_;
+};
+
 std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why) {
   // plfft_r2r_kind_t values: DCT_2 = 1, DCT_3 = 2, DST_2 = 5, DST_3 = 6
   const int k = p.r2r_kind;
@@ -257,8 +325,17 @@ std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why) {
   }
   if (!c.e) c = choose(two ? KK_DCT2_F : KK_DCT3_F, p.n, p.prec, dst);
   if (!c.e) {
-    *why = "no GPU kernel for r2r kind " + std::to_string(k) + " n=" + std::to_string(p.n);
-    return nullptr;
+    const int64_t per = p.n * 2 * prec_compute_bytes(p.prec);
+    const int64_t chunk =
+        std::max<int64_t>(1, std::min<int64_t>(p.howmany, (int64_t(64) << 20) / per));
+    const int flags = (p.adjoint ? 2 : 0) | (dst ? 4 : 0) | (two ? 0 : 8);
+    auto node = std::make_unique<DctViaComplexNode>(flags, p.n, p.prec, p.howmany, p.istride,
+                                                    id, p.ostride, od, chunk);
+    if (node->init(why) != hipSuccess) {
+      if (why->empty()) *why = "DCT fallback setup failed";
+      return nullptr;
+    }
+    return node;
   }
   auto node = std::make_unique<DctNode>(two, dst, half, c, p.n, p.prec, p.howmany, p.istride, id,
                                         p.ostride, od, p.adjoint ? 2 : 0);

@@ -34,12 +34,13 @@ enum KKind : int {
   KK_DCT2_F = 10,  // DCT-II / DST-II, full-length engine
   KK_DCT3_E = 11,  // DCT-III (bwd slot: DST-III), even n, engine length n/2
   KK_DCT3_F = 12,  // DCT-III / DST-III, full-length engine
+  KK_C2C_UNIT = 13,  // batched C2C, unit element stride, streaming access
 };
 inline const char *kkind_name(int k) {
   static const char *names[] = {"c2c",      "col",      "row",       "r2c_half", "c2r_half",
                                 "r2c_full", "c2r_full", "bluestein", "rader",    "dct2_half",
-                                "dct2_full", "dct3_half", "dct3_full"};
-  return (k >= 0 && k <= 12) ? names[k] : "?";
+                                "dct2_full", "dct3_half", "dct3_full", "c2c_unit"};
+  return (k >= 0 && k <= 13) ? names[k] : "?";
 }
 
 using KLaunch = hipError_t (*)(const KArgs &, int64_t grid, size_t lds, hipStream_t);
@@ -54,12 +55,13 @@ struct KEntry {
   std::vector<int> chain;
   int tpt, tpb, pts;
   bool uses_lds;
+  int pad = 5;  // LDS padding shift of the engine
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len
   size_t lds_bytes(int64_t len) const {
     if (!uses_lds) return 0;
-    return (size_t)tpb * (size_t)(len + (len >> 5) + 1) * 2 * prec_compute_bytes(prec);
+    return (size_t)tpb * (size_t)(len + (len >> pad) + 1) * 2 * prec_compute_bytes(prec);
   }
   // can the dynamic engine run this radix chain within pts points/thread
   // (and with the butterflies this configuration compiles)?
@@ -71,6 +73,7 @@ struct KEntry {
 const std::vector<KEntry> &kernel_registry();
 void register_kernels_f32(std::vector<KEntry> &r);
 void register_kernels_f32_4s(std::vector<KEntry> &r);
+void register_kernels_f32_unit(std::vector<KEntry> &r);
 void register_kernels_f32_real(std::vector<KEntry> &r);
 void register_kernels_f64_real(std::vector<KEntry> &r);
 void register_kernels_conv(std::vector<KEntry> &r);
@@ -88,7 +91,7 @@ hipError_t launch_chirp_in(int prec, bool bwd, const KArgs &a, hipStream_t s);
 hipError_t launch_pointwise_swap(int prec, const KArgs &a, hipStream_t s);
 hipError_t launch_chirp_out(int prec, bool bwd, const KArgs &a, hipStream_t s);
 // Real <-> complex staging (kernels_conv.hip): which = 0 widen, 1 narrow,
-// 2 hermitian, 3 realpart; `total` threads.
+// 2 hermitian, 3 realpart, 4 dct pre, 5 dct post; `total` threads.
 hipError_t launch_real_stage(int which, int prec, const KArgs &a, int64_t total, hipStream_t s);
 void register_kernels_f64(std::vector<KEntry> &r);
 void register_kernels_f16(std::vector<KEntry> &r);

@@ -215,7 +215,7 @@ class Plan:
     def __init__(self, kind: str, n: int, direction: int = PLFFT_FORWARD, dtype: str = "f32",
                  howmany: int = 1, istride: int = 1, idist: int | None = None,
                  ostride: int = 1, odist: int | None = None, may_alias: bool = False,
-                 rigor: int = PLFFT_ESTIMATE, r2r: int | str = 0):
+                 rigor: int = PLFFT_MEASURE, r2r: int | str = 0):
         lib = load()
         cfg = Config(kind, n, direction, r2r)
         in_count, out_count = _counts(kind, n)

@@ -318,9 +318,10 @@ class FourStepNode : public Node {
 };
 
 // Chunk of big transforms per column/row pair: keeps the workspace near
-// `mb` MiB (env PLFFT_FOURSTEP_CHUNK_MB overrides the default of 64).
+// `mb` MiB (env PLFFT_FOURSTEP_CHUNK_MB overrides the default of 128; on
+// MI355X smaller chunks underfill the GPU and larger ones gain nothing).
 static int64_t fourstep_chunk(int64_t n, int prec, int64_t batch) {
-  int64_t mb = 64;
+  int64_t mb = 128;
   if (const char *s = std::getenv("PLFFT_FOURSTEP_CHUNK_MB")) mb = std::max<int64_t>(1, atoll(s));
   const int64_t per = n * 2 * prec_compute_bytes(prec);
   int64_t c = std::max<int64_t>(1, (mb << 20) / per);

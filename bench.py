@@ -82,6 +82,146 @@ def cpu_baseline(n: int, seconds: float = 10.0):
                       f"({dt:.1f} s), oracle FP32 port of the reference CT plan, OpenMP"}
 
 
+def time_stream(torch, fn, steps: int, warmup: int):
+    """Mean ms per call of fn(stream) over `steps` calls, HIP events on the
+    launch stream."""
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn(stream)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn(stream)
+    end.record(stream)
+    end.synchronize()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+MIXED_COUNT, MIXED_SEED = 10000, 2024
+
+
+def progress(msg: str) -> None:
+    """Progress lines go to stderr; stdout carries only the JSON line."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_mixed(torch, P, steps: int, warmup: int):
+    """The mixed-length batch: 10,000 C2C members, N log-uniform in
+    [16, 65536], 10 % moved to a prime, packed back to back, forward,
+    out-of-place. Fused = one persistent launch (plus a counter fill);
+    plain = one ordinary plan per member, executed one after another
+    (eagerly, and captured in a graph to take host launch cost out)."""
+    import numpy as np
+
+    from plfft_amd import rng
+
+    lengths = rng.mixed_lengths(MIXED_SEED, MIXED_COUNT)
+    total = int(sum(lengths))
+    offs = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.int64)
+    flops = sum(5.0 * n * math.log2(n) for n in lengths)
+    res = {"workload": f"{MIXED_COUNT} C2C members, N log-uniform in [16, 65536], 10% moved "
+                       "to a prime, forward, out-of-place, packed",
+           "members": MIXED_COUNT, "points": total, "distinct_lengths": len(set(lengths)),
+           "primes": int(sum(rng.is_prime(n) for n in lengths))}
+    x = torch.empty(total, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+    x.view(torch.float32).uniform_(-1.0, 1.0, generator=g)
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    mp = P.MixedPlan(lengths, P.PLFFT_FORWARD, "f32")
+    res["plan_seconds"] = round(time.perf_counter() - t0, 2)
+    progress(f"mixed plan {res['plan_seconds']} s: {mp.describe()}")
+    res["plan"] = mp.describe()
+    ws = torch.empty(max(1, mp.workspace_size), dtype=torch.uint8, device="cuda")
+    fused = time_stream(torch, lambda s: mp.execute_async(x, y, stream=s, workspace=ws),
+                        steps, warmup)
+    res["fused_ms"] = round(fused, 4)
+    progress(f"mixed fused f32 {fused:.4f} ms")
+    res["fused_GFLOP/s"] = round(flops / (fused * 1e-3) / 1e9, 1)
+    gbs = 2 * total * 8 / (fused * 1e-3) / 1e9
+    res["fused_GB/s"] = round(gbs, 1)
+    res["fused_hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+
+    # accuracy of the fused f32 result: numpy FP64 FFT of every 10th member
+    torch.cuda.synchronize()
+    xh = x.cpu().numpy().astype(np.complex128)
+    yh = y.cpu().numpy().astype(np.complex128)
+    num = den = 0.0
+    worst = 0.0
+    for i in range(0, MIXED_COUNT, 10):
+        a, n = int(offs[i]), lengths[i]
+        ref = np.fft.fft(xh[a:a + n])
+        d2 = float(np.sum(np.abs(yh[a:a + n] - ref) ** 2))
+        r2 = float(np.sum(np.abs(ref) ** 2))
+        num += d2
+        den += r2
+        worst = max(worst, math.sqrt(d2 / r2))
+    res["rel_err"] = float(f"{math.sqrt(num / den):.3g}")
+    res["rel_err_max_member"] = float(f"{worst:.3g}")
+    res["rel_err_sample"] = "every 10th member vs numpy FP64 FFT"
+
+    # 16-bit storage, FP32 arithmetic
+    for dt in ("f16", "bf16"):
+        hp = P.MixedPlan(lengths, P.PLFFT_FORWARD, dt)
+        xs = torch.empty(2 * total, dtype=torch.float16 if dt == "f16" else torch.bfloat16,
+                         device="cuda")
+        xs.uniform_(-1.0, 1.0, generator=g)
+        ys = torch.empty_like(xs)
+        hws = torch.empty(max(1, hp.workspace_size), dtype=torch.uint8, device="cuda")
+        hms = time_stream(torch, lambda s: hp.execute_async(xs, ys, stream=s, workspace=hws),
+                          steps, warmup)
+        res[f"fused_{dt}_ms"] = round(hms, 4)
+        progress(f"mixed fused {dt} {hms:.4f} ms")
+        del hp, xs, ys, hws
+
+    # plain: one plan per distinct length (creation not timed)
+    t0 = time.perf_counter()
+    plans = {}
+    for i, n in enumerate(sorted(set(lengths))):
+        plans[n] = P.Plan("c2c", n, P.PLFFT_FORWARD, "f32")
+        if i % 1000 == 999:
+            progress(f"plain plans: {i + 1} in {time.perf_counter() - t0:.1f} s")
+    res["plain_plan_seconds"] = round(time.perf_counter() - t0, 2)
+    xp, yp = x.data_ptr(), y.data_ptr()
+    jobs = [(plans[n], xp + 8 * int(o), yp + 8 * int(o)) for n, o in zip(lengths, offs)]
+
+    def plain(s):
+        for pl, a, b in jobs:
+            pl.execute_async(a, b, stream=s)
+
+    psteps = max(2, min(steps, 5))
+    res["plain_eager_ms"] = round(time_stream(torch, plain, psteps, 1), 3)
+    progress(f"plain eager {res['plain_eager_ms']} ms")
+    try:
+        gs = torch.cuda.Stream()
+        gs.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(gs):
+            plain(gs)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            plain(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        res["plain_graph_ms"] = round(time_stream(torch, lambda s: graph.replay(), psteps, 1), 3)
+        del graph
+    except Exception as e:  # reported, not required
+        res["plain_graph_ms"] = None
+        res["plain_graph_error"] = str(e)[:200]
+    res["speedup_vs_plain_eager"] = round(res["plain_eager_ms"] / fused, 2)
+    if res.get("plain_graph_ms"):
+        res["speedup_vs_plain_graph"] = round(res["plain_graph_ms"] / fused, 2)
+    del plans, jobs, mp, ws, x, y
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +229,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="run only the mixed-length batch comparison")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--only", default=None,
@@ -106,6 +249,11 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+
+    if args.mixed_only:
+        print(json.dumps({"mixed": bench_mixed(torch, P, max(3, args.steps // 10),
+                                               min(args.warmup, 3))}))
+        return 0
 
     total = 1 << 27  # complex elements per case
     x = torch.empty(total, dtype=torch.complex64, device="cuda")
@@ -221,6 +369,10 @@ def main() -> int:
         "plan": plan.describe(),
         "cases": cases,
     }
+    if not args.headline_only and not args.no_mixed:
+        del x, y, xr
+        torch.cuda.empty_cache()
+        out["mixed"] = bench_mixed(torch, P, max(3, args.steps // 10), min(args.warmup, 3))
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -57,6 +57,62 @@ size_t plfft_plan_describe(const plfft_plan_t *plan, char *buf, size_t len);
 /* Text of the last plan-creation failure on this thread ("" if none). */
 const char *plfft_last_error(void);
 
+/* ------------------------------------------------------------------------
+ * Mixed-length batches: one persistent kernel launch transforms `count`
+ * complex-to-complex members of different lengths (powers of two, other
+ * smooth lengths and primes alike). The reference has no such object: a
+ * reference caller creates one plan per length (plfft_plan_create_from_config,
+ * include/plfft.h:491-501) and executes them one after another; this plan
+ * replaces that loop.
+ *
+ * Member i reads n[i] contiguous complex elements at input + in_offset[i] and
+ * writes its transform to output + out_offset[i] (offsets in complex elements
+ * of the data type; NULL packs the members back to back). Members may be
+ * transformed in place (equal offsets, input == output); member regions must
+ * not otherwise overlap. Unnormalized, like every plfft transform. Lengths up
+ * to 131072 are accepted for any n, and up to 262144 for n whose prime
+ * factors are all at most 23. data_type: FP32, FP64, FP16 or BF16 (FP32
+ * arithmetic for the 16-bit types).
+ * ---------------------------------------------------------------------- */
+typedef struct plfft_mixed_plan_t plfft_mixed_plan_t;
+
+/* Returns PLFFT_NULL_ARGUMENT (null plan or n), PLFFT_INVALID_SIZE (count < 1,
+ * a length < 1, a negative offset), PLFFT_INVALID_TRANSFORM_KIND (direction),
+ * PLFFT_INVALID_DATA_TYPE, PLFFT_INVALID_PLANNER_RIGOR, or
+ * PLFFT_PLAN_CREATION_FAILED (unsupported length or type, no device; see
+ * plfft_last_error()). */
+plfft_status_t plfft_mixed_plan_create(plfft_mixed_plan_t **plan, int64_t count,
+                                       const int64_t *n, const int64_t *in_offset,
+                                       const int64_t *out_offset,
+                                       plfft_direction_t direction,
+                                       plfft_data_type_t data_type,
+                                       plfft_rigor_t rigor);
+void plfft_mixed_plan_destroy(plfft_mixed_plan_t *plan);
+
+/* Device bytes of scratch per execution: the intermediate rows of the
+ * four-step and chirp-z members (0 when every member fits one pass). */
+size_t plfft_mixed_plan_get_workspace_size(const plfft_mixed_plan_t *plan);
+
+/* Complex elements spanned by the input (which = 0) or output (which = 1). */
+int64_t plfft_mixed_plan_get_span(const plfft_mixed_plan_t *plan, int which);
+
+/* Synchronous execute: device or host buffers (host buffers are staged).
+ * Returns PLFFT_OK or PLFFT_NULL_ARGUMENT. */
+plfft_status_t plfft_mixed_plan_execute(const plfft_mixed_plan_t *plan, const void *input,
+                                        void *output);
+
+/* Stream-ordered execute with device buffers; `workspace` as for
+ * plfft_plan_execute_async (NULL: the plan's own). Allocates nothing when
+ * `workspace` is given or the plan needs none: exactly one kernel launch, no
+ * memset, so it can be captured into a hipGraph. Returns PLFFT_OK,
+ * PLFFT_NULL_ARGUMENT, or PLFFT_PLAN_CREATION_FAILED if the launch fails. */
+plfft_status_t plfft_mixed_plan_execute_async(const plfft_mixed_plan_t *plan,
+                                              const void *input, void *output,
+                                              void *workspace, void *stream);
+
+/* Schedule summary (members per algorithm, tasks, grid, LDS, workspace). */
+size_t plfft_mixed_plan_describe(const plfft_mixed_plan_t *plan, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@
 
 #include "../../include/plfft.h"
 #include "../../include/plfft_gpu.h"
+#include "mixed_plan.hpp"
 #include "plan.hpp"
 
 namespace {
@@ -563,3 +564,174 @@ extern "C" size_t plfft_plan_describe(const plfft_plan_t *plan, char *buf, size_
 }
 
 extern "C" const char *plfft_last_error(void) { return plf::last_error(); }
+
+// ---------------------------------------------------------------------------
+// mixed-length batches (no reference counterpart: a reference caller loops
+// over one plan per length)
+// ---------------------------------------------------------------------------
+struct plfft_mixed_plan_t {
+  std::unique_ptr<plf::MixedPlan> mp;
+  int device = 0;
+  size_t elem = 0;  // bytes per complex storage element
+  mutable std::mutex ws_mu;
+  mutable void *ws_default = nullptr;
+  ~plfft_mixed_plan_t() {
+    if (ws_default) (void)hipFree(ws_default);
+  }
+};
+
+extern "C" plfft_status_t plfft_mixed_plan_create(plfft_mixed_plan_t **plan, int64_t count,
+                                                  const int64_t *n, const int64_t *in_offset,
+                                                  const int64_t *out_offset,
+                                                  plfft_direction_t direction,
+                                                  plfft_data_type_t data_type,
+                                                  plfft_rigor_t rigor) {
+  if (!plan) return PLFFT_NULL_ARGUMENT;
+  *plan = nullptr;
+  if (!n) return PLFFT_NULL_ARGUMENT;
+  if (count < 1) return PLFFT_INVALID_SIZE;
+  for (int64_t i = 0; i < count; ++i) {
+    if (n[i] < 1) return PLFFT_INVALID_SIZE;
+    if ((in_offset && in_offset[i] < 0) || (out_offset && out_offset[i] < 0))
+      return PLFFT_INVALID_SIZE;
+  }
+  if (!valid_direction(direction)) return PLFFT_INVALID_TRANSFORM_KIND;
+  int prec;
+  switch ((int)data_type) {
+    case PLFFT_DATA_TYPE_FP32: prec = plf::P_F32; break;
+    case PLFFT_DATA_TYPE_FP64: prec = plf::P_F64; break;
+    case PLFFT_DATA_TYPE_FP16: prec = plf::P_F16; break;
+    case 5 /* PLFFT_DATA_TYPE_BF16 */: prec = plf::P_BF16; break;
+    case PLFFT_DATA_TYPE_Q7:
+    case PLFFT_DATA_TYPE_Q15:
+      plf::set_last_error("mixed batch: fixed-point storage is not supported");
+      return PLFFT_PLAN_CREATION_FAILED;
+    default: return PLFFT_INVALID_DATA_TYPE;
+  }
+  if ((int)rigor < PLFFT_ESTIMATE || (int)rigor > PLFFT_EXHAUSTIVE)
+    return PLFFT_INVALID_PLANNER_RIGOR;
+  int dev = 0, ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    plf::set_last_error("no HIP device available");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  if (hipGetDevice(&dev) != hipSuccess) {
+    plf::set_last_error("hipGetDevice failed");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  std::string why;
+  plf::set_planning_rigor(rigor);
+  std::unique_ptr<plf::MixedPlan> mp =
+      plf::MixedPlan::build(prec, direction, count, n, in_offset, out_offset, &why);
+  plf::set_planning_rigor(0);
+  if (!mp) {
+    plf::set_last_error(why);
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  plfft_mixed_plan_t *pl = new (std::nothrow) plfft_mixed_plan_t;
+  if (!pl) out_of_memory();
+  pl->mp = std::move(mp);
+  pl->device = dev;
+  pl->elem = 2 * (size_t)plf::prec_real_bytes(prec);
+  plf::set_last_error("");
+  *plan = pl;
+  return PLFFT_OK;
+}
+
+extern "C" void plfft_mixed_plan_destroy(plfft_mixed_plan_t *plan) { delete plan; }
+
+extern "C" size_t plfft_mixed_plan_get_workspace_size(const plfft_mixed_plan_t *plan) {
+  return plan ? plan->mp->workspace_bytes() : 0;
+}
+
+extern "C" int64_t plfft_mixed_plan_get_span(const plfft_mixed_plan_t *plan, int which) {
+  if (!plan) return 0;
+  return which ? plan->mp->out_span() : plan->mp->in_span();
+}
+
+static void *mixed_default_ws(const plfft_mixed_plan_t *pl) {
+  if (pl->mp->workspace_bytes() == 0) return nullptr;
+  std::lock_guard<std::mutex> g(pl->ws_mu);
+  if (!pl->ws_default) {
+    if (hipMalloc(&pl->ws_default, pl->mp->workspace_bytes()) != hipSuccess) out_of_memory();
+  }
+  return pl->ws_default;
+}
+
+extern "C" plfft_status_t plfft_mixed_plan_execute_async(const plfft_mixed_plan_t *plan,
+                                                         const void *input, void *output,
+                                                         void *workspace, void *stream) {
+  if (!plan || !input || !output) return PLFFT_NULL_ARGUMENT;
+  DeviceGuard guard(plan->device);
+  void *ws = workspace ? workspace : mixed_default_ws(plan);
+  hipError_t e = plan->mp->enqueue(input, output, ws, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) {
+    plf::set_last_error(std::string("launch failed: ") + hipGetErrorString(e));
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
+  return PLFFT_OK;
+}
+
+extern "C" plfft_status_t plfft_mixed_plan_execute(const plfft_mixed_plan_t *plan,
+                                                   const void *input, void *output) {
+  if (!plan || !input || !output) return PLFFT_NULL_ARGUMENT;
+  DeviceGuard guard(plan->device);
+  const plf::MixedPlan &mp = *plan->mp;
+  const size_t ib = (size_t)mp.in_span() * plan->elem, ob = (size_t)mp.out_span() * plan->elem;
+  const bool din = is_device_pointer(input), dout = is_device_pointer(output);
+  void *ws = nullptr;
+  if (mp.workspace_bytes() && hipMalloc(&ws, mp.workspace_bytes()) != hipSuccess)
+    out_of_memory();
+  hipStream_t s = nullptr;
+  hipError_t e;
+  const void *di = input;
+  void *dout_p = output;
+  if (!din || !dout) {
+    if (input == output) {
+      // in place on the host: one staging buffer
+      const size_t b = ib > ob ? ib : ob;
+      if (hipMalloc(&dout_p, b) != hipSuccess) out_of_memory();
+      if ((e = hipMemcpy(dout_p, input, ib, hipMemcpyHostToDevice)) != hipSuccess)
+        fatal_hip("copy in", e);
+      di = dout_p;
+    } else {
+      if (!din) {
+        void *d = nullptr;
+        if (hipMalloc(&d, ib) != hipSuccess) out_of_memory();
+        if ((e = hipMemcpy(d, input, ib, hipMemcpyHostToDevice)) != hipSuccess)
+          fatal_hip("copy in", e);
+        di = d;
+      }
+      if (!dout) {
+        if (hipMalloc(&dout_p, ob) != hipSuccess) out_of_memory();
+        // gaps between members come back unchanged
+        if ((e = hipMemcpy(dout_p, output, ob, hipMemcpyHostToDevice)) != hipSuccess)
+          fatal_hip("copy out-init", e);
+      }
+    }
+  }
+  e = mp.enqueue(di, dout_p, ws, s);
+  if (e != hipSuccess) fatal_hip("kernel launch", e);
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) fatal_hip("execute", e);
+  if (!dout) {
+    if ((e = hipMemcpy(output, dout_p, ob, hipMemcpyDeviceToHost)) != hipSuccess)
+      fatal_hip("copy out", e);
+    (void)hipFree(dout_p);
+  }
+  if (!din && di != dout_p) (void)hipFree(const_cast<void *>(di));
+  if (ws) (void)hipFree(ws);
+  return PLFFT_OK;
+}
+
+extern "C" size_t plfft_mixed_plan_describe(const plfft_mixed_plan_t *plan, char *buf,
+                                            size_t len) {
+  std::string s = plan ? plan->mp->describe() : std::string("(null)");
+  if (buf && len) {
+    size_t k = s.size() < len - 1 ? s.size() : len - 1;
+    std::memcpy(buf, s.data(), k);
+    buf[k] = 0;
+  }
+  return s.size();
+}

@@ -43,14 +43,16 @@ static int64_t group_generator(int64_t n) {
   return 0;
 }
 
-// In-place forward DFT of v (interleaved FP64) on the GPU.
-static hipError_t dft_f64_gpu(std::vector<double> &v, int64_t M) {
+// In-place forward DFTs of `count` contiguous length-M rows of v
+// (interleaved FP64) on the GPU.
+hipError_t dft_f64_gpu(std::vector<double> &v, int64_t M, int64_t count) {
   Problem q;
   q.kind = K_C2C;
   q.dir = -1;
   q.prec = P_F64;
   q.n = M;
-  q.howmany = 1;
+  q.howmany = count;
+  q.idist = q.odist = M;
   q.may_alias = true;
   std::string why;
   std::unique_ptr<Node> node = build_c2c_node(q, &why);
@@ -67,7 +69,7 @@ static hipError_t dft_f64_gpu(std::vector<double> &v, int64_t M) {
 
 // Chirp c_k = exp(-i pi (k^2 mod 2n)/n), phase in exact integers
 // (bluestein.hpp:66-79).
-static std::vector<double> chirp_table(int64_t n) {
+std::vector<double> chirp_table(int64_t n) {
   std::vector<double> c(2 * (size_t)n);
   for (int64_t k = 0; k < n; ++k) {
     const int64_t ph = (int64_t)(((unsigned __int128)k * (unsigned __int128)k) %
@@ -94,7 +96,7 @@ static hipError_t bluestein_kernel(int64_t n, int64_t M, const std::vector<doubl
       (*B)[2 * (M - k) + 1] = im;
     }
   }
-  return dft_f64_gpu(*B, M);
+  return dft_f64_gpu(*B, M, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -122,7 +124,7 @@ class FusedConvNode : public Node {
         b[2 * ((m - k) % m) + 1] = -im / (double)m;
         x = x * g % n_;
       }
-      err = dft_f64_gpu(b, m);
+      err = dft_f64_gpu(b, m, 1);
       if (err == hipSuccess) err = upload_table(aux0_, b, prec_);
       if (err == hipSuccess) err = aux1_.upload(perm.data(), perm.size() * sizeof(int));
       gen_ = g;

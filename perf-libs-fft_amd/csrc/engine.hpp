@@ -130,6 +130,10 @@ struct DynamicEngine {
                                                int t, bool valid, bool lds_in, Load &load,
                                                Store &store) {
     constexpr int KMAX = (PTS + R - 1) / R;
+    // Opaque copy of the thread index: without it the compiler hoists the
+    // address arithmetic of every radix case out of the stage loop and keeps
+    // it live across all stages (150 instead of 90 VGPRs at 16 points).
+    asm volatile("" : "+v"(t));
     const int n = d->n;
     const int NB = n / R;
     const int L = d->L[s];

@@ -1,0 +1,11 @@
+// kernels_mixed_f32.hip -- persistent mixed-length kernel, FP32 storage.
+#include "kernels_mixed_impl.hpp"
+
+namespace plf {
+
+hipError_t launch_mixed_f32(const MixedArgs &a, int grid, size_t lds, hipStream_t s) {
+  return launch_mixed_t<float>(a, grid, lds, s);
+}
+int mixed_blocks_per_cu_f32(size_t lds) { return mixed_blocks_per_cu_t<float>(lds); }
+
+}  // namespace plf

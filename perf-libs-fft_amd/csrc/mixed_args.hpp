@@ -1,0 +1,47 @@
+// mixed_args.hpp -- host/device records of the mixed-length persistent launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernel_args.hpp"
+
+namespace plf {
+
+// Per-member description (device memory).
+struct MixedXf {
+  int n;         // transform length
+  int kind;      // 0 direct, 1 four-step, 2 Bluestein in one block, 3 chirp-z four-step
+  int len;       // engine length of a one-block member (n or M); M for chirp-z
+  int n1, n2;    // four-step split of n (kind 1) or of M (kind 3)
+  int d0, d1;    // engine descriptors: one-block (d0) or column (d0) / row (d1)
+  int shift;     // W_N split: N = n1*n2, W^e = hi[e >> shift] * lo[e & mask]
+  int lanes0, lanes1;  // tile width of the column / row passes (16 or 8)
+  int64_t in_off, out_off;  // complex elements
+  int64_t ws_off;           // complex elements of workspace (kind 1: N; kind 3: 2M)
+  int64_t tw4_off;          // split twiddles in aux: lo (2^shift), then hi
+  int64_t chirp_off;        // chirp c_k (n entries) in aux
+  int64_t B_off;            // B = DFT_M(b)/M (M entries) in aux
+};
+
+struct MixedArgs {
+  const void *in;
+  void *out;
+  void *ws;                // compute-precision complex workspace
+  const MixedXf *xf;
+  const int *order;        // members by decreasing predicted cost
+  int64_t count;           // members
+  const GenericDesc *descs;
+  const int64_t *desc_tw;  // twiddle offset (complex entries) of each descriptor
+  const void *tw;          // all engine twiddles
+  const void *aux;         // split twiddles, chirps, B tables
+  int bwd;                 // 1: backward transform (exponent sign +)
+};
+
+// Launch the persistent kernel for storage precision `prec` (Prec values).
+hipError_t launch_mixed(int prec, const MixedArgs &a, int grid, size_t lds, hipStream_t s);
+// Resident blocks per CU of the persistent kernel with `lds` bytes of LDS.
+int mixed_blocks_per_cu(int prec, size_t lds);
+
+}  // namespace plf

@@ -1,0 +1,46 @@
+// mixed_plan.hpp -- plan of one persistent launch over a batch of C2C
+// transforms of different lengths.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mixed_args.hpp"
+#include "plan.hpp"
+
+namespace plf {
+
+class MixedPlan {
+ public:
+  // Member i transforms n[i] complex elements read at in_off[i] and written
+  // at out_off[i] (complex elements of the storage type). A null offset
+  // array packs the members back to back. dir: -1 forward, +1 backward.
+  static std::unique_ptr<MixedPlan> build(int prec, int dir, int64_t count, const int64_t *n,
+                                          const int64_t *in_off, const int64_t *out_off,
+                                          std::string *why);
+
+  // Launch the persistent kernel on `s` (one launch, nothing else).
+  // `ws` holds workspace_bytes() of device memory.
+  hipError_t enqueue(const void *in, void *out, void *ws, hipStream_t s) const;
+  size_t workspace_bytes() const { return data_bytes_; }
+  std::string describe() const { return summary_; }
+  int prec() const { return prec_; }
+  int64_t count() const { return count_; }
+  int64_t in_span() const { return in_span_; }    // complex elements
+  int64_t out_span() const { return out_span_; }  // complex elements
+  int grid() const { return grid_; }
+  size_t lds_bytes() const { return lds_; }
+
+ private:
+  int prec_ = P_F32;
+  int bwd_ = 0;
+  int64_t count_ = 0;
+  int64_t in_span_ = 0, out_span_ = 0;
+  int grid_ = 0;
+  size_t lds_ = 0, data_bytes_ = 0;
+  DevMem xf_, order_, descs_, desc_tw_, tw_, aux_;
+  std::string summary_;
+};
+
+}  // namespace plf

@@ -108,6 +108,12 @@ std::vector<double> root_table(int64_t n, int64_t count);
 // tw[off_s + (m-1)*L + j] = exp(-2*pi*i*j*m/(L*R)), m in [1,R), j in [0,L).
 std::vector<double> stockham_twiddles(const std::vector<int> &chain);
 
+// Bluestein chirp c_k = exp(-i pi (k^2 mod 2n)/n), k < n, interleaved FP64.
+std::vector<double> chirp_table(int64_t n);
+// In-place forward DFTs of `count` contiguous length-M rows (interleaved
+// FP64) computed on the current device with this library's FP64 plans.
+hipError_t dft_f64_gpu(std::vector<double> &v, int64_t M, int64_t count);
+
 // exp(sign*2*pi*i*num/den) with num reduced exactly mod den, long double
 // evaluation rounded once to double.
 void unit_root(int64_t num, int64_t den, double *re, double *im);

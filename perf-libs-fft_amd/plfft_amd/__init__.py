@@ -70,6 +70,10 @@ EXPORTS = [
     "plfft_plan_get_workspace_size", "plfft_plan_execute_async",
     "plfft_plan_execute_adjoint", "plfft_plan_execute_adjoint_async",
     "plfft_plan_describe", "plfft_last_error",
+    "plfft_mixed_plan_create", "plfft_mixed_plan_destroy",
+    "plfft_mixed_plan_get_workspace_size", "plfft_mixed_plan_get_span",
+    "plfft_mixed_plan_execute", "plfft_mixed_plan_execute_async",
+    "plfft_mixed_plan_describe",
 ]
 
 _lib = None
@@ -125,6 +129,13 @@ def load() -> ctypes.CDLL:
         "plfft_plan_execute_adjoint_async": (i32, [vp, vp, vp, vp, vp]),
         "plfft_plan_describe": (ctypes.c_size_t, [vp, ctypes.c_char_p, ctypes.c_size_t]),
         "plfft_last_error": (ctypes.c_char_p, []),
+        "plfft_mixed_plan_create": (i32, [pp, i64, vp, vp, vp, i32, i32, i32]),
+        "plfft_mixed_plan_destroy": (None, [vp]),
+        "plfft_mixed_plan_get_workspace_size": (ctypes.c_size_t, [vp]),
+        "plfft_mixed_plan_get_span": (i64, [vp, i32]),
+        "plfft_mixed_plan_execute": (i32, [vp, vp, vp]),
+        "plfft_mixed_plan_execute_async": (i32, [vp, vp, vp, vp, vp]),
+        "plfft_mixed_plan_describe": (ctypes.c_size_t, [vp, ctypes.c_char_p, ctypes.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -264,6 +275,74 @@ class Plan:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.plfft_plan_destroy(self.h)
+            self.h = None
+
+
+class MixedPlan:
+    """A batch of C2C transforms of different lengths run as one launch
+    (plfft_mixed_plan_create of include/plfft_gpu.h).
+
+    ``lengths[i]`` complex elements of member i are read at ``in_offsets[i]``
+    and written at ``out_offsets[i]`` (complex elements; None packs the
+    members back to back).
+    """
+
+    def __init__(self, lengths, direction: int = PLFFT_FORWARD, dtype: str = "f32",
+                 in_offsets=None, out_offsets=None, rigor: int = PLFFT_MEASURE):
+        lib = load()
+        count = len(lengths)
+        arr = (ctypes.c_int64 * max(count, 1))(*[int(v) for v in lengths])
+
+        def offs(o):
+            if o is None:
+                return None
+            return (ctypes.c_int64 * max(count, 1))(*[int(v) for v in o])
+
+        self._keep = (arr, offs(in_offsets), offs(out_offsets))
+        h = ctypes.c_void_p()
+        st = lib.plfft_mixed_plan_create(
+            ctypes.byref(h), count, ctypes.cast(arr, ctypes.c_void_p),
+            ctypes.cast(self._keep[1], ctypes.c_void_p) if self._keep[1] is not None else None,
+            ctypes.cast(self._keep[2], ctypes.c_void_p) if self._keep[2] is not None else None,
+            int(direction), DATA_TYPES[dtype] if isinstance(dtype, str) else int(dtype),
+            int(rigor))
+        if st != PLFFT_OK:
+            raise PlfftError(st, "mixed plan create")
+        self.h = h
+        self.lengths = [int(v) for v in lengths]
+        self.direction, self.dtype = direction, dtype
+
+    @property
+    def in_span(self) -> int:
+        return int(load().plfft_mixed_plan_get_span(self.h, 0))
+
+    @property
+    def out_span(self) -> int:
+        return int(load().plfft_mixed_plan_get_span(self.h, 1))
+
+    @property
+    def workspace_size(self) -> int:
+        return int(load().plfft_mixed_plan_get_workspace_size(self.h))
+
+    def execute(self, x, y):
+        st = load().plfft_mixed_plan_execute(self.h, _ptr(x), _ptr(y))
+        if st != PLFFT_OK:
+            raise PlfftError(st, "mixed execute")
+
+    def execute_async(self, x, y, stream=None, workspace=None):
+        st = load().plfft_mixed_plan_execute_async(self.h, _ptr(x), _ptr(y), _ptr(workspace),
+                                                   _stream_handle(stream))
+        if st != PLFFT_OK:
+            raise PlfftError(st, "mixed execute_async")
+
+    def describe(self) -> str:
+        buf = ctypes.create_string_buffer(4096)
+        load().plfft_mixed_plan_describe(self.h, buf, 4096)
+        return buf.value.decode()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.plfft_mixed_plan_destroy(self.h)
             self.h = None
 
 

@@ -55,3 +55,39 @@ def complex_uniform(seed: int, count: int, dtype=np.complex64) -> np.ndarray:
 
 def complex_normal(seed: int, count: int, dtype=np.complex128) -> np.ndarray:
     return normal_f64(seed, 2 * count).view(np.complex128).astype(dtype)
+
+
+def is_prime(n: int) -> bool:
+    if n < 2:
+        return False
+    if n % 2 == 0:
+        return n == 2
+    f = 3
+    while f * f <= n:
+        if n % f == 0:
+            return False
+        f += 2
+    return True
+
+
+def mixed_lengths(seed: int, count: int, lo: int = 16, hi: int = 65536,
+                  prime_fraction: float = 0.1) -> list[int]:
+    """Lengths log-uniform in [lo, hi]; a `prime_fraction` of them (chosen by
+    a second uniform draw) are moved to the next prime, or to the largest
+    prime <= hi when the next one exceeds hi."""
+    u = (splitmix64(seed, 2 * count) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    out = []
+    for i in range(count):
+        v = int(round(lo * (hi / lo) ** u[2 * i]))
+        v = min(max(v, lo), hi)
+        if u[2 * i + 1] < prime_fraction:
+            p = v
+            while not is_prime(p):
+                p += 1
+            if p > hi:
+                p = hi
+                while not is_prime(p):
+                    p -= 1
+            v = p
+        out.append(v)
+    return out

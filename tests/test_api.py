@@ -194,3 +194,60 @@ def test_plan_without_gpu_fails_cleanly(lib):
     assert _plan_status(lib, h) == P.PLFFT_PLAN_CREATION_FAILED
     assert "device" in P.last_error() or P.last_error()
     lib.plfft_config_destroy(h)
+
+
+def _mixed_status(lib, lengths, direction=-1, dtype=0, rigor=1, in_offs=None, out_offs=None,
+                  null_plan=False, null_n=False):
+    count = len(lengths)
+    arr = (ctypes.c_int64 * max(count, 1))(*lengths)
+    io = (ctypes.c_int64 * max(count, 1))(*in_offs) if in_offs is not None else None
+    oo = (ctypes.c_int64 * max(count, 1))(*out_offs) if out_offs is not None else None
+    h = ctypes.c_void_p()
+    st = lib.plfft_mixed_plan_create(
+        None if null_plan else ctypes.byref(h), count,
+        None if null_n else ctypes.cast(arr, ctypes.c_void_p),
+        ctypes.cast(io, ctypes.c_void_p) if io is not None else None,
+        ctypes.cast(oo, ctypes.c_void_p) if oo is not None else None,
+        direction, dtype, rigor)
+    if st == P.PLFFT_OK:
+        lib.plfft_mixed_plan_destroy(h)
+    else:
+        assert not h.value
+    return st
+
+
+def test_mixed_plan_argument_validation(lib):
+    """Argument errors of plfft_mixed_plan_create come before any device
+    work, in this order: null pointers, sizes/offsets, direction, data
+    type, rigor."""
+    assert _mixed_status(lib, [8], null_plan=True) == P.PLFFT_NULL_ARGUMENT
+    assert _mixed_status(lib, [8], null_n=True) == P.PLFFT_NULL_ARGUMENT
+    assert _mixed_status(lib, []) == P.PLFFT_INVALID_SIZE
+    assert _mixed_status(lib, [8, 0]) == P.PLFFT_INVALID_SIZE
+    assert _mixed_status(lib, [8, -3]) == P.PLFFT_INVALID_SIZE
+    assert _mixed_status(lib, [8, 4], in_offs=[0, -1]) == P.PLFFT_INVALID_SIZE
+    assert _mixed_status(lib, [8, 4], out_offs=[-8, 0]) == P.PLFFT_INVALID_SIZE
+    assert _mixed_status(lib, [0], direction=7) == P.PLFFT_INVALID_SIZE
+    assert _mixed_status(lib, [8], direction=0) == P.PLFFT_INVALID_TRANSFORM_KIND
+    assert _mixed_status(lib, [8], direction=2, dtype=99) == P.PLFFT_INVALID_TRANSFORM_KIND
+    assert _mixed_status(lib, [8], dtype=99) == P.PLFFT_INVALID_DATA_TYPE
+    assert _mixed_status(lib, [8], dtype=-1) == P.PLFFT_INVALID_DATA_TYPE
+    assert _mixed_status(lib, [8], dtype=P.DATA_TYPES["q15"]) == P.PLFFT_PLAN_CREATION_FAILED
+    assert "fixed-point" in P.last_error()
+    assert _mixed_status(lib, [8], dtype=99, rigor=9) == P.PLFFT_INVALID_DATA_TYPE
+    assert _mixed_status(lib, [8], rigor=4) == P.PLFFT_INVALID_PLANNER_RIGOR
+    assert _mixed_status(lib, [8], rigor=-1) == P.PLFFT_INVALID_PLANNER_RIGOR
+
+
+def test_mixed_plan_without_gpu_fails_cleanly(lib):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    assert _mixed_status(lib, [8, 97, 5000]) == P.PLFFT_PLAN_CREATION_FAILED
+    assert "device" in P.last_error()
+    # null-safe queries
+    assert lib.plfft_mixed_plan_get_workspace_size(None) == 0
+    assert lib.plfft_mixed_plan_get_span(None, 0) == 0
+    assert lib.plfft_mixed_plan_execute(None, None, None) == P.PLFFT_NULL_ARGUMENT
+    assert lib.plfft_mixed_plan_execute_async(None, None, None, None, None) == P.PLFFT_NULL_ARGUMENT
+    lib.plfft_mixed_plan_destroy(None)

@@ -62,6 +62,16 @@ void orc_r2r_batched(int kind, int64_t n, int64_t howmany, int64_t istride,
 
 /* ---- fixed point (Q0.15 / Q0.7) --------------------------------------- */
 int32_t orc_q_convert(double v, int bits);
+/* Radix chain of the fixed-point transform of n (power of two); returns the
+ * number of stages, or -1 if n is not a power of two. */
+int orc_fixed_chain(int64_t n, int *chain);
+/* Fixed-point C2C of n = 2^e points, bits 15 (Q0.15) or 7 (Q0.7); x and y
+ * hold interleaved values in int32 (within the storage range). dir -1
+ * forward, +1 backward. Output = DFT(x)/n rounded stage by stage. */
+int orc_fixed_c2c(int64_t n, int dir, int bits, const int32_t *x, int32_t *y);
+/* Test hook for tests/test_oracle.py: plant a known mistake (1 truncating
+ * shifts, 2 conjugated stage twiddles; 0 = none). */
+void orc_fixed_set_plant(int mode);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,13 @@ def lib() -> ctypes.CDLL:
         L.orc_r2r_batched.argtypes = [ctypes.c_int, _i64, _i64, _i64, _i64, _i64, _i64, _dp, _dp]
         L.orc_q_convert.argtypes = [ctypes.c_double, ctypes.c_int]
         L.orc_q_convert.restype = ctypes.c_int32
+        L.orc_fixed_chain.argtypes = [_i64, ctypes.POINTER(ctypes.c_int)]
+        L.orc_fixed_chain.restype = ctypes.c_int
+        L.orc_fixed_c2c.argtypes = [_i64, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
+        L.orc_fixed_c2c.restype = ctypes.c_int
+        L.orc_fixed_set_plant.argtypes = [ctypes.c_int]
+        L.orc_fixed_set_plant.restype = None
         _lib = L
     return _lib
 
@@ -201,3 +208,27 @@ def r2r_batched(kind, x, n, howmany, istride, idist, ostride, odist, out_size) -
 
 def q_convert(v: float, bits: int) -> int:
     return int(lib().orc_q_convert(v, bits))
+
+
+def fixed_chain(n: int):
+    buf = (ctypes.c_int * 64)()
+    ns = lib().orc_fixed_chain(n, buf)
+    return None if ns < 0 else list(buf[:ns])
+
+
+def fixed_c2c(x, direction: int = -1, bits: int = 15) -> np.ndarray:
+    """Fixed-point C2C of one member. x: int array of shape (n, 2) or (2n,)
+    (re, im interleaved) within the storage range; returns int32 (n, 2)."""
+    a = np.ascontiguousarray(np.asarray(x, np.int32).reshape(-1))
+    n = a.size // 2
+    y = np.empty_like(a)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    st = lib().orc_fixed_c2c(n, direction, bits, a.ctypes.data_as(ip), y.ctypes.data_as(ip))
+    if st != 0:
+        raise ValueError(f"fixed-point C2C needs a power-of-two n and bits 15 or 7 (n={n})")
+    return y.reshape(n, 2)
+
+
+def fixed_set_plant(mode: int) -> None:
+    """Test hook: plant a known mistake in fixed_c2c (0 = none)."""
+    lib().orc_fixed_set_plant(mode)

@@ -1,0 +1,235 @@
+// fixed_plan.cpp -- plans for Q0.15 / Q0.7 transforms.
+//
+// The reference plans fixed point only for power-of-two n (no Rader or
+// Bluestein, src/planner/planner.cpp:38-47, src/make_level_data.hpp:128-130)
+// with radix-2/4/8/16 codelets (src/providers/aot/aot_kernel_data.cpp:
+// 290-295); C2C both directions, R2C forward, C2R backward
+// (scripts/gen_asm_kernels/config.py:80-97,143-164,212-237). Here C2C is one
+// kernel pass through LDS (kernels_fixed.hpp) for n <= 32768; R2C and C2R
+// run that transform on a complex staging buffer (widen, or Hermitian
+// extension, before; bins 0..n/2, or real parts, after).
+#include <algorithm>
+#include <cmath>
+#include <sstream>
+
+#include "plan.hpp"
+
+namespace plf {
+
+namespace {
+
+// src/plfft_convert.hpp:40-62: round to float, scale by 2^bits, truncate
+// toward zero with int32 saturation, then saturate to the storage width.
+int32_t q_convert(double v, int bits) {
+  const float f = (float)v;
+  const double scaled = (double)f * (double)(1LL << bits);
+  int64_t t;
+  if (scaled != scaled) t = 0;
+  else if (scaled >= 2147483647.0) t = 2147483647LL;
+  else if (scaled <= -2147483648.0) t = -2147483648LL;
+  else t = (int64_t)scaled;
+  const int64_t hi = (1LL << bits) - 1, lo = -(1LL << bits);
+  return (int32_t)std::min(hi, std::max(lo, t));
+}
+
+// Q value of exp(-2 pi i num/den), appended as a storage pair.
+void push_root(std::vector<int8_t> &tab, int bits, int64_t num, int64_t den) {
+  double re, im;
+  unit_root(num, den, &re, &im);
+  const int32_t qr = q_convert(re, bits), qi = q_convert(-im, bits);
+  if (bits == 15) {
+    const int16_t v[2] = {(int16_t)qr, (int16_t)qi};
+    const int8_t *b = reinterpret_cast<const int8_t *>(v);
+    tab.insert(tab.end(), b, b + 4);
+  } else {
+    tab.push_back((int8_t)qr);
+    tab.push_back((int8_t)qi);
+  }
+}
+
+bool pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+class FixedC2CNode : public Node {
+ public:
+  FixedC2CNode(int bits, int64_t n, bool bwd, int64_t batch, int64_t is, int64_t id, int64_t os,
+               int64_t od)
+      : bits_(bits), n_(n), bwd_(bwd), batch_(batch), is_(is), id_(id), os_(os), od_(od) {}
+
+  hipError_t init() {
+    tier_ = -1;
+    for (int t = 0; t < kNumFixedTiers; ++t)
+      if (n_ <= kFixedTiers[t].max_n) {
+        tier_ = t;
+        break;
+      }
+    if (tier_ < 0) return hipErrorInvalidValue;
+    std::vector<int> chain;
+    if (n_ > 1) chain = dynamic_chain(n_);
+    GenericDesc d{};
+    d.n = (int)n_;
+    d.ns = (int)chain.size();
+    // table: 16 codelet constants Q(W_16^k), then the stage twiddles
+    // Q(W_{LR}^{j m}) at (m-1) L + j of each stage s >= 1
+    std::vector<int8_t> tab;
+    for (int k = 0; k < 16; ++k) push_root(tab, bits_, k, 16);
+    int64_t L = 1, off = 0;
+    for (int s = 0; s < d.ns; ++s) {
+      const int R = chain[s];
+      d.radix[s] = R;
+      d.L[s] = (int)L;
+      d.twoff[s] = (int)off;
+      if (s > 0) {
+        for (int m = 1; m < R; ++m)
+          for (int64_t j = 0; j < L; ++j) push_root(tab, bits_, j * m, L * R);
+        off += L * (R - 1);
+      }
+      L *= R;
+    }
+    hipError_t err = desc_.upload(&d, sizeof(d));
+    if (err == hipSuccess) err = tab_.upload(tab.data(), tab.size());
+    return err;
+  }
+
+  hipError_t enqueue(const void *in, void *out, void *, hipStream_t s) const override {
+    return run(in, out, batch_, s);
+  }
+  // the transform of `count` members (count <= the planned batch)
+  hipError_t run(const void *in, void *out, int64_t count, hipStream_t s) const {
+    if (count <= 0) return hipSuccess;
+    const FixedTier &t = kFixedTiers[tier_];
+    KArgs a{};
+    a.in = in;
+    a.out = out;
+    a.tw = tab_.p;
+    a.desc = static_cast<const GenericDesc *>(desc_.p);
+    a.count = count;
+    a.istride = is_;
+    a.idist = id_;
+    a.ostride = os_;
+    a.odist = od_;
+    a.n = n_;
+    a.flags = bwd_ ? 1 : 0;
+    const int64_t grid = (count + t.tpb - 1) / t.tpb;
+    const size_t lds =
+        (size_t)t.tpb * (size_t)(n_ + (n_ >> 5) + 1) * (bits_ == 15 ? 4 : 2);
+    return launch_fixed(bits_, tier_, a, grid, lds, s);
+  }
+
+  std::string describe() const override {
+    const FixedTier &t = kFixedTiers[tier_];
+    std::ostringstream o;
+    o << "(fixed_q" << bits_ << "_" << (bwd_ ? "bwd" : "fwd") << " n=" << n_ << " tpt=" << t.tpt
+      << " tpb=" << t.tpb << " batch=" << batch_ << ")";
+    return o.str();
+  }
+
+ private:
+  int bits_;
+  int64_t n_;
+  bool bwd_;
+  int64_t batch_, is_, id_, os_, od_;
+  int tier_ = -1;
+  DevMem desc_, tab_;
+};
+
+// R2C (forward) or C2R (backward) through a complex staging buffer.
+class FixedRealNode : public Node {
+ public:
+  FixedRealNode(bool r2c, int bits, int64_t n, int64_t batch, int64_t is, int64_t id,
+                int64_t os, int64_t od, int64_t chunk)
+      : r2c_(r2c), bits_(bits), n_(n), batch_(batch), is_(is), id_(id), os_(os), od_(od),
+        chunk_(chunk) {}
+  hipError_t init() {
+    inner_ = std::make_unique<FixedC2CNode>(bits_, n_, !r2c_, chunk_, 1, n_, 1, n_);
+    return inner_->init();
+  }
+  size_t pair_bytes() const { return bits_ == 15 ? 4 : 2; }
+  size_t workspace_bytes() const override { return (size_t)chunk_ * (size_t)n_ * pair_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    const size_t real_b = pair_bytes() / 2;
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk_) {
+      const int64_t cb = std::min(chunk_, batch_ - b0);
+      KArgs a{};
+      a.n = n_;
+      a.count = cb;
+      a.istride = is_;
+      a.idist = id_;
+      a.ostride = os_;
+      a.odist = od_;
+      hipError_t err;
+      if (r2c_) {
+        a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * real_b;
+        a.out = work;
+        err = launch_fixed_elem(bits_, 0, a, cb * n_, s);
+      } else {
+        a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * pair_bytes();
+        a.out = work;
+        err = launch_fixed_elem(bits_, 2, a, cb * n_, s);
+      }
+      if (err == hipSuccess) err = inner_->run(work, work, cb, s);
+      if (err != hipSuccess) return err;
+      a.in = work;
+      if (r2c_) {
+        a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * pair_bytes();
+        err = launch_fixed_elem(bits_, 1, a, cb * (n_ / 2 + 1), s);
+      } else {
+        a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * real_b;
+        err = launch_fixed_elem(bits_, 3, a, cb * n_, s);
+      }
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(fixed_" << (r2c_ ? "r2c" : "c2r") << "_q" << bits_ << " n=" << n_ << " chunk=" << chunk_
+      << " " << inner_->describe() << ")";
+    return o.str();
+  }
+
+ private:
+  bool r2c_;
+  int bits_;
+  int64_t n_, batch_, is_, id_, os_, od_, chunk_;
+  std::unique_ptr<FixedC2CNode> inner_;
+};
+
+}  // namespace
+
+std::unique_ptr<Node> build_fixed_node(const Problem &p, std::string *why) {
+  if (!pow2(p.n)) {
+    *why = "fixed-point transforms need a power-of-two length";
+    return nullptr;
+  }
+  if (p.n > kFixedTiers[kNumFixedTiers - 1].max_n) {
+    *why = "fixed-point transforms are limited to n <= 32768";
+    return nullptr;
+  }
+  if (p.kind == K_R2R) {
+    *why = "fixed-point real-to-real transforms are not supported";
+    return nullptr;
+  }
+  if (p.adjoint && p.kind != K_C2C) {
+    *why = "the adjoint of a fixed-point real transform is not provided";
+    return nullptr;
+  }
+  hipError_t err;
+  if (p.kind == K_C2C) {
+    auto node = std::make_unique<FixedC2CNode>(p.fixed_bits, p.n, p.dir > 0, p.howmany,
+                                               p.istride, p.idist, p.ostride, p.odist);
+    err = node->init();
+    if (err == hipSuccess) return node;
+  } else {
+    const int64_t per = p.n * (p.fixed_bits == 15 ? 4 : 2);
+    const int64_t chunk =
+        std::max<int64_t>(1, std::min<int64_t>(p.howmany, (int64_t(128) << 20) / per));
+    auto node = std::make_unique<FixedRealNode>(p.kind == K_R2C, p.fixed_bits, p.n, p.howmany,
+                                                p.istride, p.idist, p.ostride, p.odist, chunk);
+    err = node->init();
+    if (err == hipSuccess) return node;
+  }
+  *why = std::string("fixed-point table setup failed: ") + hipGetErrorString(err);
+  return nullptr;
+}
+
+}  // namespace plf

@@ -69,6 +69,8 @@ std::unique_ptr<Node> build_node(const Problem &p, std::string *why);
 std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why);
 std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why);
 std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why);
+// Q0.15 / Q0.7 problems (p.fixed_bits != 0): power-of-two n only.
+std::unique_ptr<Node> build_fixed_node(const Problem &p, std::string *why);
 
 // ---- planner internals shared by the node builders ----------------------
 // A kernel choice for one engine length: the registry entry plus the chain.

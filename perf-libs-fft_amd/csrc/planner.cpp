@@ -384,6 +384,7 @@ std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why) {
 }
 
 std::unique_ptr<Node> build_node(const Problem &p, std::string *why) {
+  if (p.fixed_bits) return build_fixed_node(p, why);
   switch (p.kind) {
     case K_C2C:
       return build_c2c_node(p, why);

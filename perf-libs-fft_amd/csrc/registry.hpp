@@ -107,4 +107,18 @@ inline bool dynamic_radix_ok(int r) {
   return false;
 }
 
+// Fixed-point (Q0.15 / Q0.7) kernels: thread tiers of k_fixed, largest n
+// each handles in one pass through LDS.
+struct FixedTier {
+  int tpt, tpb, pts;
+  int64_t max_n;
+};
+constexpr int kNumFixedTiers = 5;
+extern const FixedTier kFixedTiers[kNumFixedTiers];
+hipError_t launch_fixed(int bits, int tier, const KArgs &a, int64_t grid, size_t lds,
+                        hipStream_t s);
+// which: 0 widen real -> complex, 1 bins 0..n/2 out, 2 Hermitian extension,
+// 3 real parts out
+hipError_t launch_fixed_elem(int bits, int which, const KArgs &a, int64_t total, hipStream_t s);
+
 }  // namespace plf

@@ -306,3 +306,103 @@ def test_fixed_point_conversion(orc):
     assert orc.q_convert(-0.7, 7) == -89
     # values are first rounded to float: 0.1 in float is 0.100000001490116
     assert orc.q_convert(0.1, 15) == int(np.float32(0.1) * 32768)
+
+
+# ---------------------------------------------------------------------------
+# fixed-point C2C (Q0.15 / Q0.7): the reference's own fixed-point checks
+# (test/delta_constant_test.hpp:40-62, tolerance 6 log2(n) LSB of
+# test/test_utils.hpp:132-151) and the mathematics of DFT/n
+# ---------------------------------------------------------------------------
+FIXED_NS = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192]
+
+
+def _fixed_tol(n):
+    return 6 * max(1, int(round(math.log2(n)))) if n > 1 else 0
+
+
+def test_fixed_chain(orc):
+    assert orc.fixed_chain(1) == []
+    assert orc.fixed_chain(2) == [2]
+    assert orc.fixed_chain(16) == [16]
+    assert orc.fixed_chain(32) == [2, 16]
+    assert orc.fixed_chain(1024) == [4, 16, 16]
+    assert orc.fixed_chain(32768) == [8, 16, 16, 16]
+    assert orc.fixed_chain(12) is None and orc.fixed_chain(0) is None
+    for n in FIXED_NS:
+        assert int(np.prod(orc.fixed_chain(n) or [1])) == n
+
+
+@pytest.mark.parametrize("bits", [15, 7])
+@pytest.mark.parametrize("n", FIXED_NS)
+def test_fixed_constant_delta_pairs(orc, bits, n):
+    one = orc.q_convert(1.0, bits)               # 2^bits - 1
+    const = np.zeros((n, 2), np.int32)
+    const[:, 0] = one
+    for d in (-1, 1):
+        y = orc.fixed_c2c(const, d, bits)        # constant -> delta of amplitude 1
+        want = np.zeros((n, 2))
+        want[0, 0] = one
+        assert np.abs(y - want).max() <= _fixed_tol(n)
+    delta = np.zeros((n, 2), np.int32)
+    delta[0, 0] = one
+    for d in (-1, 1):
+        y = orc.fixed_c2c(delta, d, bits)        # delta -> constant 1/n
+        assert np.abs(y - np.array([one / n, 0.0])).max() <= _fixed_tol(n)
+
+
+@pytest.mark.parametrize("bits", [15, 7])
+@pytest.mark.parametrize("n", [2, 8, 16, 64, 256, 1024, 4096, 16384])
+def test_fixed_matches_dft_over_n(orc, bits, n):
+    rng = np.random.default_rng(1000 * bits + n)
+    hi = 1 << bits
+    x = rng.integers(-hi, hi, size=(n, 2)).astype(np.int32)
+    z = x[:, 0] + 1j * x[:, 1]
+    for d in (-1, 1):
+        y = orc.fixed_c2c(x, d, bits)
+        ref = (np.fft.fft(z) if d < 0 else np.fft.ifft(z) * n) / n
+        err = np.concatenate([y[:, 0] - ref.real, y[:, 1] - ref.imag])
+        assert np.abs(err).max() <= _fixed_tol(n), (n, d)
+        # rounding (not truncation) at every stage: no drift of the mean
+        if n >= 64:
+            assert abs(err.mean()) < 0.25
+        # outputs stay in the storage range
+        assert y.min() >= -hi and y.max() <= hi - 1
+
+
+def test_fixed_tiny_exact(orc):
+    # n = 2: each input is shifted right by one with rounding half up
+    # (srshr #1), then added and subtracted: ((3,-3),(1,-1)) ->
+    # x' = ((2,-1),(1,0)), y = ((3,-1),(1,-1))
+    y = orc.fixed_c2c(np.array([[3, -3], [1, -1]], np.int32), -1, 15)
+    assert y.tolist() == [[3, -1], [1, -1]]
+    # saturation: the constant 32767 over 16 points sums to 32768 -> 32767
+    y = orc.fixed_c2c(np.full((16, 2), [32767, -32768], np.int32), -1, 15)
+    assert y[0].tolist() == [32767, -32768] and np.abs(y[1:]).max() == 0
+    # backward = swap o forward o swap, exactly
+    rng = np.random.default_rng(5)
+    x = rng.integers(-32768, 32768, size=(256, 2)).astype(np.int32)
+    assert np.array_equal(orc.fixed_c2c(x, 1, 15), orc.fixed_c2c(x[:, ::-1], -1, 15)[:, ::-1])
+
+
+def _fixed_checks_pass(orc, n, bits=15, seed=9):
+    rng = np.random.default_rng(seed)
+    hi = 1 << bits
+    x = rng.integers(-hi, hi, size=(n, 2)).astype(np.int32)
+    y = orc.fixed_c2c(x, -1, bits)
+    ref = np.fft.fft(x[:, 0] + 1j * x[:, 1]) / n
+    err = np.concatenate([y[:, 0] - ref.real, y[:, 1] - ref.imag])
+    return np.abs(err).max() <= _fixed_tol(n) and abs(err.mean()) < 0.25
+
+
+def test_fixed_planted_mistakes_are_caught(orc):
+    """The checks above reject plausible mistakes planted in the oracle:
+    truncating instead of rounding shifts (a drift of the mean), and
+    conjugated stage twiddles (a wrong transform)."""
+    try:
+        assert _fixed_checks_pass(orc, 4096)
+        for mode in (1, 2):
+            orc.fixed_set_plant(mode)
+            assert not _fixed_checks_pass(orc, 4096), mode
+    finally:
+        orc.fixed_set_plant(0)
+    assert _fixed_checks_pass(orc, 4096)

@@ -310,10 +310,13 @@ def main() -> int:
             ("c2c", 16384, "f32", -1, False), ("c2c", 65536, "f32", -1, False),
             ("c2c", 1 << 20, "f32", -1, False), ("c2c", 4096, "f64", -1, False),
             ("r2c", 4096, "f32", -1, False), ("c2r", 4096, "f32", 1, False),
+            ("c2c", 4096, "q15", -1, False), ("c2c", 4096, "f16", -1, False),
         ]
         for kind, cn, prec, cdir, inplace in specs:
             if prec == "f64":
                 cb = total // 2 // cn
+            elif prec in ("q15", "f16"):
+                cb = total * 2 // cn  # same bytes as the f32 cases
             elif kind in ("r2c", "c2r"):
                 cb = total // cn
             else:
@@ -328,7 +331,7 @@ def main() -> int:
                 continue
             dst = x if inplace else y
             cms = time_plan(torch, cplan, x, dst, max(10, args.steps // 2), args.warmup)
-            esz = 16 if prec == "f64" else 8
+            esz = {"f64": 16, "q15": 4, "f16": 4}.get(prec, 8)
             if kind == "c2c":
                 cbytes = 2 * cn * esz * cb
                 cfl = flops_c2c(cn, cb)

@@ -116,6 +116,127 @@ struct StaticEngine {
 };
 
 // ---------------------------------------------------------------------------
+// compile-time chain, split exchange: the same stages as StaticEngine, but
+// each LDS exchange moves the real parts, then the imaginary parts, through
+// one real array of N (padded) scalars. Half the LDS per transform lets two
+// 16384-point blocks share a CU, so one block's global traffic overlaps the
+// other's exchanges. Global input and output only (no TO_LDS / LDS_IN).
+// ---------------------------------------------------------------------------
+template <typename C, int TPT, int PAD, typename Ch>
+struct StaticEngineSplit {
+  static constexpr int N = Ch::N;
+  static constexpr int NS = Ch::count;
+  static constexpr bool USES_LDS = NS > 1;
+  template <int SI>
+  struct St {
+    static constexpr int R = Ch::radix(SI);
+    static constexpr int L = Ch::L(SI);
+    static constexpr int NB = N / R;
+    static constexpr int KB = (NB + TPT - 1) / TPT;
+    static __device__ __forceinline__ bool act(int b) { return (NB % TPT == 0) || (b < NB); }
+  };
+
+  // twiddles (all but the first stage) and butterflies of stage SI
+  template <int SI>
+  static __device__ __forceinline__ void compute(cpx<C> (&v)[St<SI>::KB][St<SI>::R],
+                                                 const cpx<C> *__restrict__ tw, int t) {
+    using T = St<SI>;
+#pragma unroll
+    for (int k = 0; k < T::KB; ++k) {
+      const int b = t + k * TPT;
+      if constexpr (SI > 0) {
+        const int j = b % T::L;
+        if (T::act(b)) {
+#pragma unroll
+          for (int m = 1; m < T::R; ++m)
+            v[k][m] = cmul(v[k][m], tw[Ch::tw_offset(SI) + (m - 1) * T::L + j]);
+        }
+      }
+      Dft<T::R, C>::run(v[k]);
+    }
+  }
+
+  // stage SI's outputs (component COMP) into LDS
+  template <int SI, int COMP>
+  static __device__ __forceinline__ void put(C *lds, const cpx<C> (&v)[St<SI>::KB][St<SI>::R],
+                                             int t) {
+    using T = St<SI>;
+#pragma unroll
+    for (int k = 0; k < T::KB; ++k) {
+      const int b = t + k * TPT;
+      if (T::act(b)) {
+        const int base = (b / T::L) * T::L * T::R + b % T::L;
+#pragma unroll
+        for (int m = 0; m < T::R; ++m)
+          lds[lds_pad<PAD>(base + m * T::L)] = COMP ? v[k][m].y : v[k][m].x;
+      }
+    }
+  }
+
+  // stage SI's inputs (component COMP) from LDS
+  template <int SI, int COMP>
+  static __device__ __forceinline__ void get(const C *lds, cpx<C> (&v)[St<SI>::KB][St<SI>::R],
+                                             int t) {
+    using T = St<SI>;
+#pragma unroll
+    for (int k = 0; k < T::KB; ++k) {
+      const int b = t + k * TPT;
+#pragma unroll
+      for (int m = 0; m < T::R; ++m) {
+        const C x = T::act(b) ? lds[lds_pad<PAD>(b + m * T::NB)] : C(0);
+        if (COMP) v[k][m].y = x;
+        else v[k][m].x = x;
+      }
+    }
+  }
+
+  template <int SI, class Store>
+  static __device__ __forceinline__ void from(cpx<C> (&v)[St<SI>::KB][St<SI>::R], C *lds,
+                                              const cpx<C> *__restrict__ tw, int t, bool valid,
+                                              Store &store) {
+    using T = St<SI>;
+    compute<SI>(v, tw, t);
+    if constexpr (SI == NS - 1) {
+#pragma unroll
+      for (int k = 0; k < T::KB; ++k) {
+        const int b = t + k * TPT;
+        if (T::act(b) && valid) {
+#pragma unroll
+          for (int m = 0; m < T::R; ++m) store(b + m * T::NB, v[k][m]);
+        }
+      }
+    } else {
+      cpx<C> w[St<SI + 1>::KB][St<SI + 1>::R];
+      put<SI, 0>(lds, v, t);
+      __syncthreads();
+      get<SI + 1, 0>(lds, w, t);
+      __syncthreads();
+      put<SI, 1>(lds, v, t);
+      __syncthreads();
+      get<SI + 1, 1>(lds, w, t);
+      if constexpr (SI + 1 < NS - 1) __syncthreads();  // the next exchange rewrites LDS
+      from<SI + 1>(w, lds, tw, t, valid, store);
+    }
+  }
+
+  template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
+  static __device__ __forceinline__ void run(cpx<C> *lds, const cpx<C> *tw, int t, bool valid,
+                                             Load &&load, Store &&store) {
+    static_assert(!TO_LDS && !LDS_IN, "split exchange: global input and output only");
+    using T = St<0>;
+    cpx<C> v[T::KB][T::R];
+#pragma unroll
+    for (int k = 0; k < T::KB; ++k) {
+      const int b = t + k * TPT;
+#pragma unroll
+      for (int m = 0; m < T::R; ++m)
+        v[k][m] = (T::act(b) && valid) ? load(b + m * T::NB) : cpx<C>{C(0), C(0)};
+    }
+    from<0>(v, reinterpret_cast<C *>(lds), tw, t, valid, store);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // runtime radix chain (descriptor in device memory). BIGP enables the
 // radix 17/19/23 butterflies, which need more registers than a 1024-thread
 // block can give each thread.

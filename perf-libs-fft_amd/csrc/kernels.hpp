@@ -26,6 +26,7 @@ struct SEngX {
   using Ch = Chain<Rs...>;
   using E = StaticEngine<C, TPT, PAD, Ch, PAIR_>;
   static constexpr int NFIX = Ch::N;
+  static constexpr bool USES_LDS = E::USES_LDS;
   static __host__ __device__ constexpr int lds_per(int64_t) { return lds_padded<PAD>(Ch::N); }
   template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
   static __device__ __forceinline__ void run(const GenericDesc *, cpx<C> *lds, const cpx<C> *tw,
@@ -36,6 +37,27 @@ struct SEngX {
 template <typename C_, int TPT_, int... Rs>
 using SEng = SEngX<C_, TPT_, 5, false, Rs...>;
 
+// compile-time chain with the split (real, then imaginary) LDS exchange:
+// half the LDS of SEngX; lds_per counts complex slots
+template <typename C_, int TPT_, int PAD_, int... Rs>
+struct SEngS {
+  using C = C_;
+  static constexpr int TPT = TPT_;
+  static constexpr int PAD = PAD_;
+  using Ch = Chain<Rs...>;
+  using E = StaticEngineSplit<C, TPT, PAD, Ch>;
+  static constexpr int NFIX = Ch::N;
+  static constexpr bool USES_LDS = E::USES_LDS;
+  static __host__ __device__ constexpr int lds_per(int64_t) {
+    return (lds_padded<PAD>(Ch::N) + 1) / 2;
+  }
+  template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
+  static __device__ __forceinline__ void run(const GenericDesc *, cpx<C> *lds, const cpx<C> *tw,
+                                             int t, bool valid, Load &&load, Store &&store) {
+    E::template run<TO_LDS, LDS_IN>(lds, tw, t, valid, load, store);
+  }
+};
+
 template <typename C_, int TPT_, int PTS_, int TPB_>
 struct DEng {
   using C = C_;
@@ -43,6 +65,7 @@ struct DEng {
   static constexpr int PTS = PTS_;
   static constexpr int PAD = 5;
   static constexpr int NFIX = 0;
+  static constexpr bool USES_LDS = true;
   static constexpr bool BIGP = TPT_ <= 256 && TPB_ <= 16 && TPT_ * TPB_ <= 256;
   using E = DynamicEngine<C, TPT, PTS, BIGP>;
   static __host__ __device__ constexpr int lds_per(int64_t n) { return lds_padded<PAD>((int)n); }

@@ -20,6 +20,12 @@ void register_kernels_f32_unit(std::vector<KEntry> &r) {
   PLF_UNIT(S, P, 256,   2, 4, true,  8, 8, 8, 8);
   PLF_UNIT(S, P, 1024,  1, 4, false, 16, 16, 16, 4);
   PLF_UNIT(S, P, 512,   1, 4, true,  16, 16, 16, 4);
+  // split (real, then imaginary) LDS exchange: two 16384-point blocks per CU
+  PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 16, 16, 16, 4);
+  PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 4, 16, 16, 16);
+  PLF_UNIT_SPLIT(S, P, 512, 1, 5, 16, 16, 8, 4);
+  PLF_UNIT_SPLIT(S, P, 256, 1, 5, 16, 16, 16);
+  PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 16, 16, 16, 8);
 }
 
 }  // namespace plf

@@ -53,27 +53,50 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even(KArgs a) {
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(m);
   const int64_t is = a.istride, os = a.ostride;
   const bool pair_ok = a.flags & 1;
-  Eng::template run<true>(
-      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
-      [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, [&](int, cpx<C>) {});
-  if (!valid) return;
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
   // flags bit1: interior bins doubled (the adjoint of C2R)
   const C h = (a.flags & 2) ? C(1) : C(0.5);
-  for (int k = t; 2 * k <= m; k += TPT) {
+  auto post = [&](int k, cpx<C> w) {
     if (k == 0) {
       const cpx<C> z0 = lds[lds_pad<Eng::PAD>(0)];
       out[0] = ST::st({z0.x + z0.y, C(0)});
       out[(int64_t)m * os] = ST::st({z0.x - z0.y, C(0)});
-      continue;
+      return;
     }
     const cpx<C> A = lds[lds_pad<Eng::PAD>(k)];
     const cpx<C> B = lds[lds_pad<Eng::PAD>(m - k)];
     const cpx<C> xe = {h * (A.x + B.x), h * (A.y - B.y)};
     const cpx<C> xo = {h * (A.y + B.y), h * (B.x - A.x)};  // -i (A - conj B) / 2
-    const cpx<C> wx = cmul(W[k], xo);
+    const cpx<C> wx = cmul(w, xo);
     out[(int64_t)k * os] = ST::st(cadd(xe, wx));
     if (2 * k != m) out[(int64_t)(m - k) * os] = ST::st({xe.x - wx.x, wx.y - xe.y});
+  };
+  constexpr int NF = Eng::NFIX;
+  if constexpr (NF > 0) {
+    // compile-time length: the post-twiddles are fetched before the FFT so
+    // that their latency hides behind it, and the epilogue is unrolled
+    constexpr int KP = (NF / 2 + TPT) / TPT;  // ceil((NF/2 + 1) / TPT)
+    cpx<C> wk[KP];
+#pragma unroll
+    for (int i = 0; i < KP; ++i) {
+      const int k = t + i * TPT;
+      wk[i] = (2 * k <= NF) ? W[k] : cpx<C>{C(0), C(0)};
+    }
+    Eng::template run<true>(
+        a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+        [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, [&](int, cpx<C>) {});
+    if (!valid) return;
+#pragma unroll
+    for (int i = 0; i < KP; ++i) {
+      const int k = t + i * TPT;
+      if (2 * k <= NF) post(k, wk[i]);
+    }
+  } else {
+    Eng::template run<true>(
+        a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+        [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, [&](int, cpx<C>) {});
+    if (!valid) return;
+    for (int k = t; 2 * k <= m; k += TPT) post(k, W[k]);
   }
 }
 
@@ -100,6 +123,15 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
   const bool pair_ok = a.flags & 1;
   const C hw = (a.flags & 2) ? C(0.5) : C(1);  // adjoint of R2C: interior bins halved
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
+  auto store = [&](int e, cpx<C> v) {
+    v = cswap(v);
+    if (pair_ok) {
+      reinterpret_cast<E *>(y)[e] = ST::st(v);
+    } else {
+      y[(int64_t)(2 * e) * os] = ST::str(v.x);
+      y[(int64_t)(2 * e + 1) * os] = ST::str(v.y);
+    }
+  };
   Eng::template run<false>(
       a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
       [&](int e) {
@@ -118,15 +150,7 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
         const cpx<C> zp = {s.x - wd.y, s.y + wd.x};  // s + i wd
         return cswap(zp);                           // inverse DFT via swap
       },
-      [&](int e, cpx<C> v) {
-        v = cswap(v);
-        if (pair_ok) {
-          reinterpret_cast<E *>(y)[e] = ST::st(v);
-        } else {
-          y[(int64_t)(2 * e) * os] = ST::str(v.x);
-          y[(int64_t)(2 * e + 1) * os] = ST::str(v.y);
-        }
-      });
+      store);
 }
 
 // ---------------------------------------------------------------------------

@@ -113,3 +113,17 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
     r[r.size() - 1].pad = PAD;                                                            \
     r[r.size() - 2].pad = PAD;                                                            \
   } while (0)
+
+// Unit-stride C2C on the split-exchange engine (half the LDS).
+#define PLF_UNIT_SPLIT(S, PREC, TPT, TPB, PAD, ...)                                       \
+  do {                                                                                    \
+    using EngT = ::plf::SEngS<typename ::plf::StorageT<S>::C, TPT, PAD, __VA_ARGS__>;      \
+    ::plf::add_entry(r, ::plf::KK_C2C_UNIT, PREC, ::plf::Chain<__VA_ARGS__>::N,              \
+                     {__VA_ARGS__}, TPT, TPB, 0, ::plf::Chain<__VA_ARGS__>::count > 1,     \
+                     &::plf::launch_k<&::plf::k_c2c_unit<S, false, TPB, EngT>, TPT * TPB>, \
+                     &::plf::launch_k<&::plf::k_c2c_unit<S, true, TPB, EngT>, TPT * TPB>); \
+    for (int i = 1; i <= 2; ++i) {                                                        \
+      r[r.size() - i].pad = PAD;                                                          \
+      r[r.size() - i].lds_scalars = 1;                                                    \
+    }                                                                                     \
+  } while (0)

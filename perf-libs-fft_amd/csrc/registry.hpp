@@ -56,12 +56,13 @@ struct KEntry {
   int tpt, tpb, pts;
   bool uses_lds;
   int pad = 5;  // LDS padding shift of the engine
+  int lds_scalars = 2;  // LDS scalars per element: 2, or 1 for the split exchange
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len
   size_t lds_bytes(int64_t len) const {
     if (!uses_lds) return 0;
-    return (size_t)tpb * (size_t)(len + (len >> pad) + 1) * 2 * prec_compute_bytes(prec);
+    return (size_t)tpb * (size_t)(len + (len >> pad) + 1) * lds_scalars * prec_compute_bytes(prec);
   }
   // can the dynamic engine run this radix chain within pts points/thread
   // (and with the butterflies this configuration compiles)?

@@ -58,9 +58,10 @@ size_t plfft_plan_describe(const plfft_plan_t *plan, char *buf, size_t len);
 const char *plfft_last_error(void);
 
 /* ------------------------------------------------------------------------
- * Mixed-length batches: one persistent kernel launch transforms `count`
- * complex-to-complex members of different lengths (powers of two, other
- * smooth lengths and primes alike). The reference has no such object: a
+ * Mixed-length batches: one plan transforms `count` complex-to-complex
+ * members of different lengths (powers of two, other smooth lengths and
+ * primes alike) in at most three kernel launches, one per dependency level
+ * of the multi-pass members. The reference has no such object: a
  * reference caller creates one plan per length (plfft_plan_create_from_config,
  * include/plfft.h:491-501) and executes them one after another; this plan
  * replaces that loop.
@@ -103,8 +104,8 @@ plfft_status_t plfft_mixed_plan_execute(const plfft_mixed_plan_t *plan, const vo
 
 /* Stream-ordered execute with device buffers; `workspace` as for
  * plfft_plan_execute_async (NULL: the plan's own). Allocates nothing when
- * `workspace` is given or the plan needs none: exactly one kernel launch, no
- * memset, so it can be captured into a hipGraph. Returns PLFFT_OK,
+ * `workspace` is given or the plan needs none: kernel launches only (at
+ * most three), no memset, so it can be captured into a hipGraph. Returns PLFFT_OK,
  * PLFFT_NULL_ARGUMENT, or PLFFT_PLAN_CREATION_FAILED if the launch fails. */
 plfft_status_t plfft_mixed_plan_execute_async(const plfft_mixed_plan_t *plan,
                                               const void *input, void *output,

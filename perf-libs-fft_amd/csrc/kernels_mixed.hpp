@@ -1,22 +1,19 @@
-// kernels_mixed.hpp -- one persistent launch for a batch of mixed lengths.
+// kernels_mixed.hpp -- a batch of mixed lengths in at most three launches.
 //
-// Every member is transformed by exactly one block of 256 threads, from its
-// first read to its last write; blocks never wait for each other, so the
-// launch needs no atomics, counters or inter-block synchronization. The host
-// sorts the members by predicted cost (mixed_plan.cpp) and block b takes the
-// members at positions b, 2G-1-b, 2G+b, 4G-1-b, ... of that order (G blocks),
-// which balances the blocks to within about one member.
-//
-// A member runs one of four schedules:
-//   direct      the whole member in LDS (runtime-chain engine, 256 threads)
-//   Bluestein   chirp, DFT_M, pointwise B, DFT_M, chirp, all in LDS
-//   four-step   column tiles (16 or 8 columns per pass through LDS, twiddled
-//               into the workspace), then row tiles with a transposed store
-//   chirp-z     Bluestein whose two DFT_M are four-step passes through the
-//               workspace (chirp-in columns, pointwise rows into a natural-
-//               order staging half, columns, chirp-out rows)
-// The passes of one member are separated by block barriers, which also make
-// the block's workspace writes visible to its own later reads.
+// Members are split into tasks of one 256-thread block each:
+//   ONE       a whole member in LDS: direct (runtime-chain engine) or
+//             Bluestein (chirp, DFT_M, pointwise B, DFT_M, chirp)
+//   COL/ROW   four-step tiles: 16 (or 8) columns per pass through LDS,
+//             twiddled into the member's workspace; then 16 (or 8) rows
+//             with a transposed store
+//   CZ_*      chirp-z (Bluestein whose DFT_M are four-step): chirp-in
+//             column tiles; row tiles that finish DFT_M, multiply by B and
+//             start the inverse DFT_M in LDS; chirp-out column tiles
+// A launch runs one dependency level (ONE + COL + CZ_IN, then ROW + CZ_MID,
+// then CZ_OUT); stream order separates the levels, so blocks never wait for
+// each other and need no atomics. Within a launch the host orders the tasks
+// by predicted cost and block b takes positions b, 2G-1-b, 2G+b, ... (snake
+// order over G blocks).
 #pragma once
 
 #include "engine.hpp"
@@ -35,7 +32,7 @@ struct MixedExec {
   static constexpr int PAD = 5;
 
   // pass kinds of the tile functions
-  enum Pass : int { COL = 0, ROW = 1, CZ_COL1 = 2, CZ_ROW1 = 3, CZ_COL2 = 4, CZ_ROW2 = 5 };
+  enum Pass : int { COL = 0, ROW = 1, CZ_IN = 2, CZ_OUT = 3 };
 
   // backward = swap o forward o swap (swap exchanges re and im)
   static __device__ __forceinline__ cpx<C> io(const MixedArgs &a, cpx<C> v) {
@@ -102,17 +99,25 @@ struct MixedExec {
     cpx<C> *ws = static_cast<cpx<C> *>(a.ws) + x.ws_off;
     const E *in = static_cast<const E *>(a.in) + x.in_off;
     const cpx<C> *chirp = aux + x.chirp_off;
-    const int64_t M = (int64_t)n1 * n2;
+    if (pass == CZ_OUT) {
+      // last pass of chirp-z: DFT_n1 down column m2 of the workspace gives
+      // bins m = m2 + n2 * m1; unswap, multiply by the chirp, store m < n
+      E *out = static_cast<E *>(a.out) + x.out_off;
+      EngT::template run<false>(
+          d, lds, tw, t, valid, [&](int e) { return ws[col + (int64_t)e * n2]; },
+          [&](int m1, cpx<C> v) {
+            const int64_t m = col + (int64_t)n2 * m1;
+            if (m < x.n) out[m] = ST::st(io(a, cmul(cswap(v), chirp[m])));
+          });
+      return;
+    }
     EngT::template run<false>(
         d, lds, tw, t, valid,
         [&](int e) {
           const int64_t j = col + (int64_t)e * n2;
           if (pass == COL) return io(a, ST::ld(in[j]));
-          if (pass == CZ_COL1) {
-            if (j >= x.n) return cpx<C>{C(0), C(0)};
-            return cmul(io(a, ST::ld(in[j])), chirp[j]);
-          }
-          return ws[M + j];  // CZ_COL2: staged natural-order data
+          if (j >= x.n) return cpx<C>{C(0), C(0)};  // CZ_IN: chirp, zero padding
+          return cmul(io(a, ST::ld(in[j])), chirp[j]);
         },
         [&](int k1, cpx<C> v) {
           ws[(int64_t)k1 * n2 + col] = cmul(v, split_tw(aux, x, (int64_t)col * k1));
@@ -121,7 +126,7 @@ struct MixedExec {
 
   // row pass over rows [first, first + LANES), transposed store through LDS
   template <class EngT, int LANES>
-  static __device__ __forceinline__ void row_tile(const MixedArgs &a, int pass, int first,
+  static __device__ __forceinline__ void row_tile(const MixedArgs &a, int first,
                                                   const MixedXf &x, cpx<C> *lds_all) {
     constexpr int TPT = 256 / LANES;
     const int g = threadIdx.x / TPT;
@@ -133,8 +138,7 @@ struct MixedExec {
     cpx<C> *lds = lds_all + g * lper;
     const GenericDesc *d = a.descs + x.d1;
     const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw) + a.desc_tw[x.d1];
-    const cpx<C> *aux = static_cast<const cpx<C> *>(a.aux);
-    cpx<C> *ws = static_cast<cpx<C> *>(a.ws) + x.ws_off;
+    const cpx<C> *ws = static_cast<const cpx<C> *>(a.ws) + x.ws_off;
     const cpx<C> *src = ws + (int64_t)(valid ? row : 0) * n2;
     EngT::template run<true>(
         d, lds, tw, t, valid, [&](int e) { return src[e]; }, [&](int, cpx<C>) {});
@@ -142,66 +146,74 @@ struct MixedExec {
     const int rc = first + c;
     if (rc >= n1) return;
     const cpx<C> *l = lds_all + c * lper;
-    const int64_t M = (int64_t)n1 * n2;
     E *out = static_cast<E *>(a.out) + x.out_off;
-    const cpx<C> *B = aux + x.B_off;
-    const cpx<C> *chirp = aux + x.chirp_off;
-    for (int k2 = threadIdx.x / LANES; k2 < n2; k2 += TPT) {
-      const cpx<C> v = l[lds_pad<PAD>(k2)];
-      const int64_t k = rc + (int64_t)n1 * k2;
-      if (pass == ROW) {
-        out[k] = ST::st(io(a, v));
-      } else if (pass == CZ_ROW1) {
-        ws[M + k] = cswap(cmul(v, B[k]));
-      } else if (k < x.n) {  // CZ_ROW2
-        out[k] = ST::st(io(a, cmul(cswap(v), chirp[k])));
-      }
-    }
+    for (int k2 = threadIdx.x / LANES; k2 < n2; k2 += TPT)
+      out[rc + (int64_t)n1 * k2] = ST::st(io(a, l[lds_pad<PAD>(k2)]));
   }
 
-  // All column tiles (or all row tiles) of one member. Each tile ends with a
-  // barrier: the next tile rewrites the LDS this one read, and a row pass
-  // reads what the column pass before it wrote.
-  static __device__ __forceinline__ void cols(const MixedArgs &a, int pass, const MixedXf &x,
-                                              cpx<C> *lds) {
-    for (int c = 0; c < x.n2; c += x.lanes0) {
-      if (x.lanes0 == 16) col_tile<Eng16, 16>(a, pass, c, x, lds);
-      else col_tile<Eng8, 8>(a, pass, c, x, lds);
-      __syncthreads();
-    }
-  }
-  static __device__ __forceinline__ void rows(const MixedArgs &a, int pass, const MixedXf &x,
-                                              cpx<C> *lds) {
-    for (int r = 0; r < x.n1; r += x.lanes1) {
-      if (x.lanes1 == 16) row_tile<Eng16, 16>(a, pass, r, x, lds);
-      else row_tile<Eng8, 8>(a, pass, r, x, lds);
-      __syncthreads();
-    }
+  // Middle pass of chirp-z, rows [first, first + LANES) of the workspace.
+  // Row k1 holds column-pass outputs; its DFT_n2 gives A[k1 + n1 k2] in LDS.
+  // The inverse transform of A B (as swap o DFT o swap) splits the same way
+  // with the roles of the sides exchanged: its first pass is a DFT_n2 over
+  // that same row, so the row is multiplied by B, swapped, transformed again
+  // in LDS, twiddled by W_M^{k1 m2} and written back in place.
+  template <class EngT, int LANES>
+  static __device__ __forceinline__ void row_mid(const MixedArgs &a, int first, const MixedXf &x,
+                                                 cpx<C> *lds_all) {
+    constexpr int TPT = 256 / LANES;
+    const int g = threadIdx.x / TPT;
+    const int t = threadIdx.x % TPT;
+    const int n1 = x.n1, n2 = x.n2;
+    const int row = first + g;
+    const bool valid = row < n1;
+    cpx<C> *lds = lds_all + g * lds_padded<PAD>(n2);
+    const GenericDesc *d = a.descs + x.d1;
+    const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw) + a.desc_tw[x.d1];
+    const cpx<C> *aux = static_cast<const cpx<C> *>(a.aux);
+    const cpx<C> *B = aux + x.B_off;
+    cpx<C> *rowp = static_cast<cpx<C> *>(a.ws) + x.ws_off + (int64_t)(valid ? row : 0) * n2;
+    EngT::template run<true>(
+        d, lds, tw, t, valid, [&](int e) { return rowp[e]; }, [&](int, cpx<C>) {});
+    if (valid)
+      for (int k2 = t; k2 < n2; k2 += TPT) {
+        const int p = lds_pad<PAD>(k2);
+        lds[p] = cswap(cmul(lds[p], B[row + (int64_t)n1 * k2]));
+      }
+    __syncthreads();
+    EngT::template run<false, true>(
+        d, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; },
+        [&](int m2, cpx<C> v) { rowp[m2] = cmul(v, split_tw(aux, x, (int64_t)row * m2)); });
   }
 
   static __device__ __forceinline__ void run(const MixedArgs &a, cpx<C> *lds) {
     const int G = gridDim.x, b = blockIdx.x;
     for (int r = 0;; ++r) {
       const int64_t p = (int64_t)r * G + ((r & 1) ? G - 1 - b : b);
-      if (p >= a.count) break;
-      const MixedXf x = a.xf[a.order[p]];
-      switch (x.kind) {
-        case 0:
-        case 2:
+      if (p >= a.ntasks) break;
+      const MixedTask tk = a.tasks[p];
+      const MixedXf x = a.xf[tk.xf];
+      switch (tk.type) {
+        case MT_ONE:
           one_block(a, x, lds);
-          __syncthreads();
           break;
-        case 1:
-          cols(a, COL, x, lds);
-          rows(a, ROW, x, lds);
+        case MT_COL:
+        case MT_CZ_IN:
+        case MT_CZ_OUT: {
+          const int pass = tk.type == MT_COL ? COL : (tk.type == MT_CZ_IN ? CZ_IN : CZ_OUT);
+          if (x.lanes0 == 16) col_tile<Eng16, 16>(a, pass, tk.first, x, lds);
+          else col_tile<Eng8, 8>(a, pass, tk.first, x, lds);
           break;
-        default:
-          cols(a, CZ_COL1, x, lds);
-          rows(a, CZ_ROW1, x, lds);
-          cols(a, CZ_COL2, x, lds);
-          rows(a, CZ_ROW2, x, lds);
+        }
+        case MT_ROW:
+          if (x.lanes1 == 16) row_tile<Eng16, 16>(a, tk.first, x, lds);
+          else row_tile<Eng8, 8>(a, tk.first, x, lds);
+          break;
+        default:  // MT_CZ_MID
+          if (x.lanes1 == 16) row_mid<Eng16, 16>(a, tk.first, x, lds);
+          else row_mid<Eng8, 8>(a, tk.first, x, lds);
           break;
       }
+      __syncthreads();  // the next task rewrites the LDS this one read
     }
   }
 };

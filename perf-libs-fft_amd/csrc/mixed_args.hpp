@@ -19,10 +19,25 @@ struct MixedXf {
   int shift;     // W_N split: N = n1*n2, W^e = hi[e >> shift] * lo[e & mask]
   int lanes0, lanes1;  // tile width of the column / row passes (16 or 8)
   int64_t in_off, out_off;  // complex elements
-  int64_t ws_off;           // complex elements of workspace (kind 1: N; kind 3: 2M)
+  int64_t ws_off;           // complex elements of workspace (kind 1: N; kind 3: M)
   int64_t tw4_off;          // split twiddles in aux: lo (2^shift), then hi
   int64_t chirp_off;        // chirp c_k (n entries) in aux
   int64_t B_off;            // B = DFT_M(b)/M (M entries) in aux
+};
+
+enum MixedTaskType : int {
+  MT_ONE = 0,     // whole member in one pass through LDS
+  MT_COL = 1,     // four-step column tile
+  MT_ROW = 2,     // four-step row tile
+  MT_CZ_IN = 3,   // chirp-z: chirp-in column tile
+  MT_CZ_MID = 4,  // chirp-z: DFT, x B, inverse-DFT row tile
+  MT_CZ_OUT = 5,  // chirp-z: chirp-out column tile
+};
+
+struct MixedTask {
+  int type;
+  int xf;     // member
+  int first;  // first column / row of the tile
 };
 
 struct MixedArgs {
@@ -30,8 +45,8 @@ struct MixedArgs {
   void *out;
   void *ws;                // compute-precision complex workspace
   const MixedXf *xf;
-  const int *order;        // members by decreasing predicted cost
-  int64_t count;           // members
+  const MixedTask *tasks;  // this launch's tasks, by decreasing predicted cost
+  int64_t ntasks;
   const GenericDesc *descs;
   const int64_t *desc_tw;  // twiddle offset (complex entries) of each descriptor
   const void *tw;          // all engine twiddles

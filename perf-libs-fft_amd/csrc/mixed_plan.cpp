@@ -2,14 +2,15 @@
 //
 // The reference executes a batch as one plan per length (src/plfft.cpp:
 // 283-330 builds a plan for a single n; a caller with many lengths loops over
-// plans). Here the whole batch is one kernel launch in which each member is
-// transformed by one block of 256 threads (kernels_mixed.hpp).
+// plans). Here the whole batch runs as at most three launches, one per
+// dependency level of the multi-pass schedules (kernels_mixed.hpp).
 // Each member gets one of four schedules:
 //   direct      smooth n whose radix chain fits one block (n <= 4096)
 //   four-step   smooth n above that: column tiles, then row tiles, through
 //               a workspace region of its own
 //   Bluestein   other n with a padded length M <= 4096: one block
-//   chirp-z     other n: Bluestein around two four-step transforms of M
+//   chirp-z     other n: Bluestein around two four-step transforms of M,
+//               fused into three passes
 // (the same algorithms as the single-length planner: Stockham passes,
 // src/planner/planner.cpp:38-47 for the factor choice, and Bluestein with
 // the chirp phase k^2 mod 2n in integers, src/bluestein.hpp:66-98).
@@ -75,24 +76,33 @@ struct Split {
   int l0 = 0, l1 = 0;
 };
 
-// The split N = n1 * n2 with the least padded tile work, then the most
+// Predicted cost of the two tile passes of a split, in element units: each
+// tile (one block task) costs a fixed overhead plus the elements it moves.
+constexpr int64_t kTileOverhead = 4096;
+int64_t split_cost(const Split &sp) {
+  const int64_t ncol = (sp.n2 + sp.l0 - 1) / sp.l0, nrow = (sp.n1 + sp.l1 - 1) / sp.l1;
+  return ncol * (kTileOverhead + sp.l0 * sp.n1) + nrow * (kTileOverhead + sp.l1 * sp.n2);
+}
+
+// The split N = n1 * n2 with the least predicted tile cost, then the most
 // balanced one.
 bool best_split(int64_t N, Split *out) {
   bool found = false;
-  int64_t best_work = 0, best_gap = 0;
+  int64_t best_cost = 0, best_gap = 0;
   for (int64_t n1 = 2; n1 <= kMaxSide && n1 < N; ++n1) {
     if (N % n1) continue;
     const int64_t n2 = N / n1;
     if (n2 > kMaxSide || n2 < 2) continue;
     const int l0 = lanes_for(n1), l1 = lanes_for(n2);
     if (!l0 || !l1) continue;
-    const int64_t work = (n2 + l0 - 1) / l0 * l0 * n1 + (n1 + l1 - 1) / l1 * l1 * n2;
+    const Split sp{n1, n2, l0, l1};
+    const int64_t cost = split_cost(sp);
     const int64_t gap = std::llabs(n1 - n2);
-    if (!found || work < best_work || (work == best_work && gap < best_gap)) {
+    if (!found || cost < best_cost || (cost == best_cost && gap < best_gap)) {
       found = true;
-      best_work = work;
+      best_cost = cost;
       best_gap = gap;
-      *out = Split{n1, n2, l0, l1};
+      *out = sp;
     }
   }
   return found;
@@ -121,12 +131,23 @@ bool schedule_for(int64_t n, Schedule *s, std::string *why) {
       s->len = M;
       return true;
     }
-  for (int64_t M = std::max<int64_t>(2 * n - 1, kOneBlock + 1); M <= kMaxSide * kMaxSide; ++M)
-    if (smooth7(M) && best_split(M, &s->sp)) {
+  // chirp-z: among the 7-smooth M >= 2n-1 up to 1/8 above the first one
+  // that splits, the one with the least predicted cost (three tile passes)
+  int64_t first_M = 0, best_cost = 0;
+  for (int64_t M = std::max<int64_t>(2 * n - 1, kOneBlock + 1); M <= kMaxSide * kMaxSide; ++M) {
+    if (first_M && M > first_M + first_M / 8) break;
+    Split sp;
+    if (!smooth7(M) || !best_split(M, &sp)) continue;
+    if (!first_M) first_M = M;
+    const int64_t cost = split_cost(sp) + sp.n1 * sp.n2;  // + the middle pass's second DFT
+    if (s->kind != 3 || cost < best_cost) {
       s->kind = 3;
       s->len = M;
-      return true;
+      s->sp = sp;
+      best_cost = cost;
     }
+  }
+  if (s->kind == 3) return true;
   std::ostringstream o;
   o << "mixed batch: no schedule for length " << n;
   *why = o.str();
@@ -248,10 +269,14 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, co
     }
   }
 
-  // members, their workspace and their predicted cost (HBM elements moved
-  // plus an LDS-work term)
+  // members, their workspace, and their tasks per dependency level with a
+  // predicted cost (HBM elements moved plus an LDS-work term)
   std::vector<MixedXf> xfs((size_t)count);
-  std::vector<double> cost((size_t)count);
+  struct HostTask {
+    MixedTask t;
+    double cost;
+  };
+  std::vector<HostTask> level[3];
   int64_t ws = 0, ipos = 0, opos = 0;
   size_t lds = 0;
   int nkind[4] = {0, 0, 0, 0};
@@ -273,11 +298,12 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, co
     x.chirp_off = chirp_of.count(nn) ? chirp_of[nn] : 0;
     x.B_off = B_of.count(nn) ? B_of[nn] : 0;
     ++nkind[s.kind];
-    const double lwork = 0.1 * (double)s.len * std::log2((double)std::max<int64_t>(2, s.len));
+    const int xi = (int)i;
     if (s.kind == 0 || s.kind == 2) {
       x.d0 = get_desc(s.len);
       lds = std::max(lds, (size_t)padded(s.len) * csize);
-      cost[(size_t)i] = 2.0 * (double)nn + lwork;
+      const double lwork = 0.1 * (double)s.len * std::log2((double)std::max<int64_t>(2, s.len));
+      level[0].push_back({MixedTask{MT_ONE, xi, 0}, 2.0 * (double)nn + lwork});
       continue;
     }
     const Split &sp = s.sp;
@@ -291,21 +317,32 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, co
     x.tw4_off = spl.first;
     x.shift = spl.second;
     x.ws_off = ws;
-    ws += s.kind == 1 ? s.len : 2 * s.len;
+    ws += s.len;
     lds = std::max(lds, (size_t)sp.l0 * padded(sp.n1) * csize);
     lds = std::max(lds, (size_t)sp.l1 * padded(sp.n2) * csize);
-    cost[(size_t)i] = s.kind == 1 ? 4.0 * (double)nn + lwork : 8.0 * (double)s.len + 2 * lwork;
+    const double ccost = sp.l0 * (double)sp.n1 * (2.0 + 0.1 * std::log2((double)sp.n1));
+    const double rcost = sp.l1 * (double)sp.n2 * (2.0 + 0.1 * std::log2((double)sp.n2));
+    const bool cz = s.kind == 3;
+    for (int64_t c = 0; c < sp.n2; c += sp.l0) {
+      level[0].push_back({MixedTask{cz ? MT_CZ_IN : MT_COL, xi, (int)c}, ccost});
+      if (cz) level[2].push_back({MixedTask{MT_CZ_OUT, xi, (int)c}, ccost});
+    }
+    for (int64_t r = 0; r < sp.n1; r += sp.l1)
+      level[1].push_back({MixedTask{cz ? MT_CZ_MID : MT_ROW, xi, (int)r}, cz ? 2 * rcost : rcost});
   }
-  // the most expensive first; blocks take them in snake order
-  std::vector<int> order((size_t)count);
-  for (int64_t i = 0; i < count; ++i) order[(size_t)i] = (int)i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int a, int b) { return cost[(size_t)a] > cost[(size_t)b]; });
+  std::vector<MixedTask> tasks;
+  for (int L = 0; L < 3; ++L) {
+    std::stable_sort(level[L].begin(), level[L].end(),
+                     [](const HostTask &a, const HostTask &b) { return a.cost > b.cost; });
+    P.level_first_[L] = (int64_t)tasks.size();
+    P.level_count_[L] = (int64_t)level[L].size();
+    for (const HostTask &h : level[L]) tasks.push_back(h.t);
+  }
   P.data_bytes_ = (size_t)ws * csize;
   P.lds_ = std::max<size_t>(lds, 16);
 
   hipError_t err = P.xf_.upload(xfs.data(), xfs.size() * sizeof(MixedXf));
-  if (err == hipSuccess) err = P.order_.upload(order.data(), order.size() * sizeof(int));
+  if (err == hipSuccess) err = P.tasks_.upload(tasks.data(), tasks.size() * sizeof(MixedTask));
   if (err == hipSuccess) err = P.descs_.upload(descs.data(), descs.size() * sizeof(GenericDesc));
   if (err == hipSuccess) err = P.desc_tw_.upload(desc_tw.data(), desc_tw.size() * sizeof(int64_t));
   if (err == hipSuccess) err = upload_table(P.tw_, tw, prec);
@@ -324,11 +361,12 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, co
     *why = "mixed batch: the kernel does not fit a CU with this LDS size";
     return nullptr;
   }
-  P.grid_ = (int)std::min<int64_t>(count, (int64_t)per_cu * cus);
+  P.grid_ = per_cu * cus;
 
   std::ostringstream o;
   o << "(mixed count=" << count << " direct=" << nkind[0] << " fourstep=" << nkind[1]
     << " bluestein=" << nkind[2] << " chirpz=" << nkind[3] << " lengths=" << sched.size()
+    << " tasks=" << P.level_count_[0] << "+" << P.level_count_[1] << "+" << P.level_count_[2]
     << " grid=" << P.grid_ << " lds=" << P.lds_
     << " workspace=" << P.workspace_bytes() << ")";
   P.summary_ = o.str();
@@ -336,20 +374,25 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, co
 }
 
 hipError_t MixedPlan::enqueue(const void *in, void *out, void *ws, hipStream_t s) const {
-  if (count_ == 0) return hipSuccess;
   MixedArgs a{};
   a.in = in;
   a.out = out;
   a.ws = ws;
   a.xf = static_cast<const MixedXf *>(xf_.p);
-  a.order = static_cast<const int *>(order_.p);
-  a.count = count_;
   a.descs = static_cast<const GenericDesc *>(descs_.p);
   a.desc_tw = static_cast<const int64_t *>(desc_tw_.p);
   a.tw = tw_.p;
   a.aux = aux_.p;
   a.bwd = bwd_;
-  return launch_mixed(prec_, a, grid_, lds_, s);
+  for (int L = 0; L < 3; ++L) {
+    if (level_count_[L] == 0) continue;
+    a.tasks = static_cast<const MixedTask *>(tasks_.p) + level_first_[L];
+    a.ntasks = level_count_[L];
+    const int grid = (int)std::min<int64_t>(level_count_[L], grid_);
+    const hipError_t err = launch_mixed(prec_, a, grid, lds_, s);
+    if (err != hipSuccess) return err;
+  }
+  return hipSuccess;
 }
 
 }  // namespace plf

@@ -20,8 +20,9 @@ class MixedPlan {
                                           const int64_t *in_off, const int64_t *out_off,
                                           std::string *why);
 
-  // Launch the persistent kernel on `s` (one launch, nothing else).
-  // `ws` holds workspace_bytes() of device memory.
+  // Launch the kernel on `s` once per non-empty dependency level (at most
+  // three launches, nothing else). `ws` holds workspace_bytes() of device
+  // memory.
   hipError_t enqueue(const void *in, void *out, void *ws, hipStream_t s) const;
   size_t workspace_bytes() const { return data_bytes_; }
   std::string describe() const { return summary_; }
@@ -37,9 +38,10 @@ class MixedPlan {
   int bwd_ = 0;
   int64_t count_ = 0;
   int64_t in_span_ = 0, out_span_ = 0;
-  int grid_ = 0;
+  int grid_ = 0;  // resident blocks (largest grid of a launch)
+  int64_t level_first_[3] = {0, 0, 0}, level_count_[3] = {0, 0, 0};
   size_t lds_ = 0, data_bytes_ = 0;
-  DevMem xf_, order_, descs_, desc_tw_, tw_, aux_;
+  DevMem xf_, tasks_, descs_, desc_tw_, tw_, aux_;
   std::string summary_;
 };
 

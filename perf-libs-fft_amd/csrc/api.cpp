@@ -501,15 +501,24 @@ static const plf::Node *adjoint_node(const plfft_plan_t *pl) {
         q.dir = PLFFT_FORWARD;
         q.adjoint = 1;
         break;
-      case plf::K_R2R:  // DCT-II <-> DCT-III, DST-II <-> DST-III with end weights
+      case plf::K_R2R:
+        // DCT-II <-> DCT-III and DST-II <-> DST-III with end weights;
+        // R2HC <-> HC2R with interior weights; DCT-I with interior weights;
+        // DCT-IV, DST-I, DST-IV and DHT are symmetric (their own adjoint)
         switch (q.r2r_kind) {
-          case PLFFT_R2R_DCT_2: q.r2r_kind = PLFFT_R2R_DCT_3; break;
-          case PLFFT_R2R_DCT_3: q.r2r_kind = PLFFT_R2R_DCT_2; break;
-          case PLFFT_R2R_DST_2: q.r2r_kind = PLFFT_R2R_DST_3; break;
-          case PLFFT_R2R_DST_3: q.r2r_kind = PLFFT_R2R_DST_2; break;
+          case PLFFT_R2R_DCT_2: q.r2r_kind = PLFFT_R2R_DCT_3; q.adjoint = 1; break;
+          case PLFFT_R2R_DCT_3: q.r2r_kind = PLFFT_R2R_DCT_2; q.adjoint = 1; break;
+          case PLFFT_R2R_DST_2: q.r2r_kind = PLFFT_R2R_DST_3; q.adjoint = 1; break;
+          case PLFFT_R2R_DST_3: q.r2r_kind = PLFFT_R2R_DST_2; q.adjoint = 1; break;
+          case PLFFT_R2R_R2HC: q.r2r_kind = PLFFT_R2R_HC2R; q.adjoint = 1; break;
+          case PLFFT_R2R_HC2R: q.r2r_kind = PLFFT_R2R_R2HC; q.adjoint = 1; break;
+          case PLFFT_R2R_DCT_1: q.adjoint = 1; break;
+          case PLFFT_R2R_DCT_4:
+          case PLFFT_R2R_DST_1:
+          case PLFFT_R2R_DST_4:
+          case PLFFT_R2R_DHT: break;
           default: ok = false;
         }
-        q.adjoint = 1;
         break;
       default:
         ok = false;

@@ -71,7 +71,9 @@ static hipError_t real_stage_t(int which, const KArgs &a, int64_t total, hipStre
     case 2: hipLaunchKernelGGL((k_hermitian<S>), grid, dim3(256), 0, s, a); break;
     case 3: hipLaunchKernelGGL((k_realpart<S>), grid, dim3(256), 0, s, a); break;
     case 4: hipLaunchKernelGGL((k_dct_pre<S>), grid, dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_dct_post<S>), grid, dim3(256), 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_dct_post<S>), grid, dim3(256), 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_r2r_pre<S>), grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_r2r_post<S>), grid, dim3(256), 0, s, a); break;
   }
   return hipGetLastError();
 }

@@ -194,4 +194,113 @@ __global__ void __launch_bounds__(256) k_dct_post(KArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Remaining R2R kinds (FFTW definitions, docs/plfft.md:186-230) through one
+// complex DFT of length L per member: a pre kernel builds the complex input
+// in a staging buffer (count x L, compute precision), a post kernel maps the
+// transform to the real output. KArgs: n = logical length, n1 = L,
+// flags = r2r kind | 256 (adjoint weights), aux0 / aux1 = pre / post
+// rotation tables (DCT-IV / DST-IV), strides and distances of the real
+// sides.
+//   DCT-I  (0): L = 2(n-1), even extension;           y_k = Re V_k
+//   DCT-IV (3): L = 2n, x_j e^{-i pi j/2n}, zero pad;  y_k = 2 Re(q_k V_k)
+//   DST-I  (4): L = 2(n+1), odd extension;            y_k = -Im V_{k+1}
+//   DST-IV (7): as DCT-IV;                             y_k = -2 Im(q_k V_k)
+//   DHT    (8): L = n, (x, 0);                         y_k = Re V_k - Im V_k
+//   R2HC   (9): L = n, (x, 0);   y_k = Re V_k (k <= n/2), Im V_{n-k} (k > n/2)
+//   HC2R  (10): L = n, Hermitian unpack, backward DFT; y_j = Re v_j
+// with q_k = e^{-i pi (2k+1)/4n}. Adjoint weights: DCT-I halves interior
+// inputs and doubles interior outputs; HC2R (the adjoint of R2HC) halves
+// interior inputs; R2HC (the adjoint of HC2R) doubles interior outputs.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(256) k_r2r_pre(KArgs a) {
+  using ST = StorageT<S>;
+  using C = typename ST::C;
+  using Rs = typename ST::Rs;
+  const int64_t n = a.n, L = a.n1;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.count * L) return;
+  const int64_t b = i / L, j = i % L;
+  const Rs *x = static_cast<const Rs *>(a.in) + b * a.idist;
+  const int kind = a.flags & 255;
+  const bool adj = a.flags & 256;
+  auto ld = [&](int64_t k) { return ST::ldr(x[k * a.istride]); };
+  cpx<C> v = {C(0), C(0)};
+  switch (kind) {
+    case 0: {  // DCT-I
+      const int64_t k = j < n ? j : L - j;
+      v.x = ld(k);
+      if (adj && k != 0 && k != n - 1) v.x *= C(0.5);
+      break;
+    }
+    case 3:
+    case 7:  // DCT-IV / DST-IV
+      if (j < n) {
+        const C xv = ld(j);
+        const cpx<C> p = static_cast<const cpx<C> *>(a.aux0)[j];
+        v = {xv * p.x, xv * p.y};
+      }
+      break;
+    case 4:  // DST-I
+      if (j >= 1 && j <= n) v.x = ld(j - 1);
+      else if (j >= n + 2) v.x = -ld(2 * n + 1 - j);
+      break;
+    case 10: {  // HC2R: X_k = (h_k, h_{n-k}), X_{n-k} = conj X_k
+      const C w = (adj && j != 0 && 2 * j != n) ? C(0.5) : C(1);
+      if (j == 0) v = {ld(0), C(0)};
+      else if (2 * j == n) v = {w * ld(j), C(0)};
+      else if (2 * j < n) v = {w * ld(j), w * ld(n - j)};
+      else v = {w * ld(n - j), -w * ld(j)};
+      v = cswap(v);  // backward DFT via swap
+      break;
+    }
+    default:  // DHT, R2HC
+      v.x = ld(j);
+      break;
+  }
+  static_cast<cpx<C> *>(a.out)[i] = v;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_r2r_post(KArgs a) {
+  using ST = StorageT<S>;
+  using C = typename ST::C;
+  using Rs = typename ST::Rs;
+  const int64_t n = a.n, L = a.n1;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.count * n) return;
+  const int64_t b = i / n, k = i % n;
+  const cpx<C> *V = static_cast<const cpx<C> *>(a.in) + b * L;
+  const int kind = a.flags & 255;
+  const bool adj = a.flags & 256;
+  C y;
+  switch (kind) {
+    case 0:  // DCT-I
+      y = V[k].x;
+      if (adj && k != 0 && k != n - 1) y *= C(2);
+      break;
+    case 3:
+    case 7: {  // DCT-IV / DST-IV
+      const cpx<C> r = cmul(static_cast<const cpx<C> *>(a.aux1)[k], V[k]);
+      y = kind == 3 ? C(2) * r.x : C(-2) * r.y;
+      break;
+    }
+    case 4:  // DST-I
+      y = -V[k + 1].y;
+      break;
+    case 8:  // DHT
+      y = V[k].x - V[k].y;
+      break;
+    case 9:  // R2HC
+      y = 2 * k <= n ? V[k].x : V[n - k].y;
+      if (adj && k != 0 && 2 * k != n) y *= C(2);
+      break;
+    default:  // HC2R: real part of the swapped-back result
+      y = V[k].y;
+      break;
+  }
+  static_cast<Rs *>(a.out)[b * a.odist + k * a.ostride] = ST::str(y);
+}
+
 }  // namespace plf

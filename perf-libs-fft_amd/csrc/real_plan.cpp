@@ -307,12 +307,113 @@ class DctViaComplexNode : public Node {
   DevMem w_;
 };
 
+// DCT-I, DCT-IV, DST-I, DST-IV, DHT, R2HC and HC2R through one complex DFT
+// of length L per member (pre and post kernels k_r2r_pre / k_r2r_post,
+// launch_real_stage 6 / 7), in chunks of members.
+class R2RViaComplexNode : public Node {
+ public:
+  R2RViaComplexNode(int kind, bool adjoint, int64_t n, int prec, int64_t batch, int64_t is,
+                    int64_t id, int64_t os, int64_t od)
+      : kind_(kind), adj_(adjoint), n_(n), prec_(prec), batch_(batch), is_(is), id_(id),
+        os_(os), od_(od) {
+    L_ = kind == 0 ? 2 * (n - 1) : kind == 4 ? 2 * (n + 1) : (kind == 3 || kind == 7) ? 2 * n : n;
+    const int64_t per = L_ * 2 * prec_compute_bytes(prec);
+    chunk_ = std::max<int64_t>(1, std::min<int64_t>(batch, (int64_t(64) << 20) / per));
+  }
+  hipError_t init(std::string *why) {
+    Problem q;
+    q.kind = K_C2C;
+    q.dir = -1;
+    q.prec = prec_compute_bytes(prec_) == 8 ? P_F64 : P_F32;
+    q.n = L_;
+    q.howmany = chunk_;
+    q.istride = q.ostride = 1;
+    q.idist = q.odist = L_;
+    q.may_alias = true;
+    inner_ = build_c2c_node(q, why);
+    if (!inner_) return hipErrorInvalidValue;
+    if (kind_ != 3 && kind_ != 7) return hipSuccess;
+    // p_j = e^{-i pi j / 2n} = W_4n^j, q_k = e^{-i pi (2k+1) / 4n} = W_8n^{2k+1}
+    std::vector<double> qk(2 * (size_t)n_);
+    for (int64_t k = 0; k < n_; ++k) {
+      double re, im;
+      unit_root(2 * k + 1, 8 * n_, &re, &im);
+      qk[2 * k] = re;
+      qk[2 * k + 1] = -im;
+    }
+    hipError_t err = upload_table(pre_, root_table(4 * n_, n_), prec_);
+    if (err == hipSuccess) err = upload_table(post_, qk, prec_);
+    return err;
+  }
+  size_t buf_bytes() const { return (size_t)chunk_ * (size_t)L_ * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return buf_bytes() + inner_->workspace_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    char *buf = static_cast<char *>(work);
+    void *iws = buf + buf_bytes();
+    const size_t rb = (size_t)prec_real_bytes(prec_);
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk_) {
+      const int64_t cb = std::min(chunk_, batch_ - b0);
+      KArgs a{};
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * rb;
+      a.out = buf;
+      a.aux0 = pre_.p;
+      a.aux1 = post_.p;
+      a.count = cb;
+      a.istride = is_;
+      a.idist = id_;
+      a.ostride = os_;
+      a.odist = od_;
+      a.n = n_;
+      a.n1 = L_;
+      a.flags = kind_ | (adj_ ? 256 : 0);
+      hipError_t err = launch_real_stage(6, prec_, a, cb * L_, s);
+      if (err == hipSuccess) err = inner_->enqueue(buf, buf, iws, s);
+      if (err != hipSuccess) return err;
+      a.in = buf;
+      a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * rb;
+      err = launch_real_stage(7, prec_, a, cb * n_, s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    static const char *names[] = {"dct1", "dct2", "dct3", "dct4", "dst1", "dst2",
+                                  "dst3", "dst4", "dht",  "r2hc", "hc2r"};
+    std::ostringstream o;
+    o << "(" << names[kind_] << (adj_ ? "_adjoint" : "") << "_via_c2c n=" << n_ << " L=" << L_
+      << " chunk=" << chunk_ << " batch=" << batch_ << " " << inner_->describe() << ")";
+    return o.str();
+  }
+
+ private:
+  int kind_;
+  bool adj_;
+  int64_t n_;
+  int prec_;
+  int64_t batch_, is_, id_, os_, od_;
+  int64_t L_ = 0, chunk_ = 1;
+  std::unique_ptr<Node> inner_;
+  DevMem pre_, post_;
+};
+
 std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why) {
-  // plfft_r2r_kind_t values: DCT_2 = 1, DCT_3 = 2, DST_2 = 5, DST_3 = 6
+  // plfft_r2r_kind_t values: DCT_1 = 0, DCT_2 = 1, DCT_3 = 2, DCT_4 = 3,
+  // DST_1 = 4, DST_2 = 5, DST_3 = 6, DST_4 = 7, DHT = 8, R2HC = 9, HC2R = 10
   const int k = p.r2r_kind;
-  if (k != 1 && k != 2 && k != 5 && k != 6) {
-    *why = "r2r kind " + std::to_string(k) + " has no GPU implementation yet";
+  if (k < 0 || k > 10) {
+    *why = "unknown r2r kind " + std::to_string(k);
     return nullptr;
+  }
+  if (k != 1 && k != 2 && k != 5 && k != 6) {
+    const int64_t id = p.howmany > 1 ? p.idist : 0;
+    const int64_t od = p.howmany > 1 ? p.odist : 0;
+    auto node = std::make_unique<R2RViaComplexNode>(k, p.adjoint != 0, p.n, p.prec, p.howmany,
+                                                    p.istride, id, p.ostride, od);
+    if (node->init(why) != hipSuccess) {
+      if (why->empty()) *why = "r2r setup failed";
+      return nullptr;
+    }
+    return node;
   }
   const bool two = (k == 1 || k == 5), dst = (k == 5 || k == 6);
   const int64_t id = p.howmany > 1 ? p.idist : 0;

@@ -40,9 +40,10 @@ struct StaticEngine {
   static constexpr int LDS_PER = lds_padded<PAD>(N);
   static constexpr bool USES_LDS = NS > 1;
 
-  template <int SI, bool TO_LDS, bool LDS_IN, class Load, class Store>
+  template <int SI, bool TO_LDS, bool LDS_IN, class Load, class Store, bool REGS_IN = false>
   static __device__ __forceinline__ void stage(cpx<C> *lds, const cpx<C> *__restrict__ tw,
-                                               int t, bool valid, Load &load, Store &store) {
+                                               int t, bool valid, Load &load, Store &store,
+                                               const cpx<C> *vin = nullptr) {
     constexpr int R = Ch::radix(SI);
     constexpr int L = Ch::L(SI);
     constexpr int NB = N / R;
@@ -57,7 +58,9 @@ struct StaticEngine {
       const bool act = (NB % TPT == 0) || (b < NB);
 #pragma unroll
       for (int m = 0; m < R; ++m) {
-        if constexpr (FIRST && !LDS_IN) {
+        if constexpr (FIRST && REGS_IN) {
+          v[k][m] = vin[k * R + m];
+        } else if constexpr (FIRST && !LDS_IN) {
           v[k][m] = (act && valid) ? load(b + m * NB) : cpx<C>{C(0), C(0)};
         } else {
           v[k][m] = act ? lds[lds_pad<PAD>(b + m * NB)] : cpx<C>{C(0), C(0)};
@@ -112,6 +115,16 @@ struct StaticEngine {
   static __device__ __forceinline__ void run(cpx<C> *lds, const cpx<C> *tw, int t, bool valid,
                                              Load &&load, Store &&store) {
     stage<0, TO_LDS, LDS_IN>(lds, tw, t, valid, load, store);
+  }
+
+  // Stage 0 takes its inputs from registers: vin[k * R0 + m] holds element
+  // b + m * (N / R0) of butterfly b = t + k * TPT (zero where b >= N / R0).
+  // Lets a caller issue the next member's loads before this member's stages.
+  template <class Store>
+  static __device__ __forceinline__ void run_regs(const cpx<C> *vin, cpx<C> *lds,
+                                                  const cpx<C> *tw, int t, Store &&store) {
+    auto none = [](int) { return cpx<C>{C(0), C(0)}; };
+    stage<0, false, false, decltype(none), Store, true>(lds, tw, t, true, none, store, vin);
   }
 };
 

@@ -133,6 +133,53 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2c_unit(KArgs a) {
       [&](int e, cpx<C> v) { ST::stnt(out + e, BWD ? cswap(v) : v); });
 }
 
+// batched C2C, unit element stride, software-pipelined: a block walks the
+// members blockIdx.x, +gridDim.x, ... and issues the stage-0 loads of its
+// next member before running the LDS stages of the current one, so each
+// block keeps global loads in flight through its own exchanges. Launched
+// with a grid of at most (CUs x resident blocks per CU).
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT) k_c2c_pipe(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  using Ch = typename Eng::Ch;
+  constexpr int TPT = Eng::TPT;
+  constexpr int N = Ch::N;
+  constexpr int R0 = Ch::radix(0);
+  constexpr int NB0 = N / R0;
+  constexpr int KB0 = (NB0 + TPT - 1) / TPT;
+  const int t = threadIdx.x;
+  cpx<C> *lds = dyn_lds<C>();
+  const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw);
+  cpx<C> nxt[KB0 * R0];
+  auto fetch = [&](int64_t mem) {
+    const E *in = static_cast<const E *>(a.in) + mem * a.idist;
+#pragma unroll
+    for (int k = 0; k < KB0; ++k) {
+      const int b = t + k * TPT;
+      const bool act = (NB0 % TPT == 0) || (b < NB0);
+#pragma unroll
+      for (int m = 0; m < R0; ++m) {
+        cpx<C> v = act ? ST::ldnt(in + b + m * NB0) : cpx<C>{C(0), C(0)};
+        nxt[k * R0 + m] = BWD ? cswap(v) : v;
+      }
+    }
+  };
+  int64_t mem = blockIdx.x;
+  if (mem < a.count) fetch(mem);
+  for (; mem < a.count; mem += gridDim.x) {
+    cpx<C> cur[KB0 * R0];
+#pragma unroll
+    for (int i = 0; i < KB0 * R0; ++i) cur[i] = nxt[i];
+    if (mem + gridDim.x < a.count) fetch(mem + gridDim.x);
+    E *out = static_cast<E *>(a.out) + mem * a.odist;
+    Eng::E::run_regs(cur, lds, tw, t,
+                     [&](int e, cpx<C> v) { ST::stnt(out + e, BWD ? cswap(v) : v); });
+    __syncthreads();  // the next member's stage 0 rewrites this member's LDS
+  }
+}
+
 // W_N^e from the split table: hi[e >> shift] * lo[e & (2^shift - 1)].
 template <typename C>
 __device__ __forceinline__ cpx<C> split_twiddle(const cpx<C> *__restrict__ lo,

@@ -20,6 +20,11 @@ void register_kernels_f32_unit(std::vector<KEntry> &r) {
   PLF_UNIT(S, P, 256,   2, 4, true,  8, 8, 8, 8);
   PLF_UNIT(S, P, 1024,  1, 4, false, 16, 16, 16, 4);
   PLF_UNIT(S, P, 512,   1, 4, true,  16, 16, 16, 4);
+  // software-pipelined persistent blocks (next member's loads in flight);
+  // measured within a few percent of the plain 4096 kernel, slower for
+  // shorter and longer lengths, so only these remain as tuning candidates
+  PLF_PIPE(S, P, 256, 4, 4, 16, 16, 16);
+  PLF_PIPE(S, P, 256, 4, 3, 16, 16, 16);
   // split (real, then imaginary) LDS exchange: two 16384-point blocks per CU
   PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 16, 16, 16, 4);
   PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 4, 16, 16, 16);

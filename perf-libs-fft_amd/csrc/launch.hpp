@@ -1,6 +1,7 @@
 // launch.hpp -- launch wrappers and registration helpers for kernels.hpp.
 #pragma once
 
+#include <algorithm>
 #include <initializer_list>
 
 #include "kernels.hpp"
@@ -18,6 +19,19 @@ hipError_t launch_k(const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
   }
   hipLaunchKernelGGL(K, dim3((unsigned)grid), dim3(BLOCK), lds, s, a);
   return hipGetLastError();
+}
+
+// Persistent launch: at most `per_cu` blocks per compute unit.
+template <void (*K)(KArgs), int BLOCK, int PER_CU>
+hipError_t launch_pipe(const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
+  static const int cus = [] {
+    int d = 0, c = 0;
+    if (hipGetDevice(&d) != hipSuccess ||
+        hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess)
+      return 256;
+    return c > 0 ? c : 256;
+  }();
+  return launch_k<K, BLOCK>(a, std::min<int64_t>(grid, (int64_t)cus * PER_CU), lds, s);
 }
 
 inline void add_entry1(std::vector<KEntry> &r, int kind, int prec, int n,
@@ -112,6 +126,20 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
                      &::plf::launch_k<&::plf::k_c2c_unit<S, true, TPB, EngT>, TPT * TPB>); \
     r[r.size() - 1].pad = PAD;                                                            \
     r[r.size() - 2].pad = PAD;                                                            \
+  } while (0)
+
+// Unit-stride software-pipelined C2C (k_c2c_pipe), PER_CU blocks per CU.
+#define PLF_PIPE(S, PREC, TPT, PAD, PER_CU, ...)                                           \
+  do {                                                                                    \
+    using EngT = ::plf::SEngX<typename ::plf::StorageT<S>::C, TPT, PAD, false, __VA_ARGS__>; \
+    ::plf::add_entry(r, ::plf::KK_C2C_UNIT, PREC, ::plf::Chain<__VA_ARGS__>::N,              \
+                     {__VA_ARGS__}, TPT, 1, 0, true,                                      \
+                     &::plf::launch_pipe<&::plf::k_c2c_pipe<S, false, 1, EngT>, TPT, PER_CU>, \
+                     &::plf::launch_pipe<&::plf::k_c2c_pipe<S, true, 1, EngT>, TPT, PER_CU>); \
+    for (int i = 1; i <= 2; ++i) {                                                        \
+      r[r.size() - i].pad = PAD;                                                          \
+      r[r.size() - i].persist = PER_CU;                                                   \
+    }                                                                                     \
   } while (0)
 
 // Unit-stride C2C on the split-exchange engine (half the LDS).

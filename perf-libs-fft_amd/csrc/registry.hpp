@@ -57,6 +57,7 @@ struct KEntry {
   bool uses_lds;
   int pad = 5;  // LDS padding shift of the engine
   int lds_scalars = 2;  // LDS scalars per element: 2, or 1 for the split exchange
+  int persist = 0;      // > 0: persistent launch with at most this many blocks per CU
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len
@@ -92,7 +93,8 @@ hipError_t launch_chirp_in(int prec, bool bwd, const KArgs &a, hipStream_t s);
 hipError_t launch_pointwise_swap(int prec, const KArgs &a, hipStream_t s);
 hipError_t launch_chirp_out(int prec, bool bwd, const KArgs &a, hipStream_t s);
 // Real <-> complex staging (kernels_conv.hip): which = 0 widen, 1 narrow,
-// 2 hermitian, 3 realpart, 4 dct pre, 5 dct post; `total` threads.
+// 2 hermitian, 3 realpart, 4 dct pre, 5 dct post, 6 r2r pre, 7 r2r post
+// (the other R2R kinds); `total` threads.
 hipError_t launch_real_stage(int which, int prec, const KArgs &a, int64_t total, hipStream_t s);
 void register_kernels_f64(std::vector<KEntry> &r);
 void register_kernels_f16(std::vector<KEntry> &r);

@@ -311,6 +311,7 @@ def main() -> int:
             ("c2c", 1 << 20, "f32", -1, False), ("c2c", 4096, "f64", -1, False),
             ("r2c", 4096, "f32", -1, False), ("c2r", 4096, "f32", 1, False),
             ("c2c", 4096, "q15", -1, False), ("c2c", 4096, "f16", -1, False),
+            ("r2r", 4096, "f32", -1, False),
         ]
         for kind, cn, prec, cdir, inplace in specs:
             if prec == "f64":
@@ -324,7 +325,8 @@ def main() -> int:
             if cn == 1024 and kind == "c2c" and prec == "f32":
                 cb = 4096  # the reference's CPU-runnable configuration
             try:
-                cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace)
+                cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace,
+                               r2r="dct2")
             except P.PlfftError as e:
                 cases.append({"kind": kind, "n": cn, "dtype": prec, "batch": cb,
                               "status": f"unsupported: {e}"})
@@ -335,11 +337,23 @@ def main() -> int:
             if kind == "c2c":
                 cbytes = 2 * cn * esz * cb
                 cfl = flops_c2c(cn, cb)
+            elif kind == "r2r":  # DCT-II: n reals in, n reals out
+                cbytes = 2 * cn * (esz // 2) * cb
+                cfl = flops_c2c(cn, cb) / 2
             else:
                 cbytes = (cn * esz // 2 + (cn // 2 + 1) * esz) * cb
                 cfl = flops_c2c(cn, cb) / 2
             gbs = cbytes / (cms * 1e-3) / 1e9
+            adj = None
+            if not inplace and prec == "f32" and cn == 4096:
+                # adjoint / forward time (the adjoint kernels reuse the forward ones)
+                ams = time_plan(torch, cplan, y, x, max(10, args.steps // 2), args.warmup,
+                                adjoint=True)
+                adj = round(ams / cms, 3)
+                xr.uniform_(-1.0, 1.0, generator=g)  # the adjoint wrote into x
+                torch.cuda.synchronize()
             cases.append({"kind": kind, "n": cn, "dtype": prec, "batch": cb,
+                          "adjoint_over_forward": adj,
                           "dir": "fwd" if cdir < 0 else "bwd",
                           "placement": "in-place" if inplace else "out-of-place",
                           "ms": round(cms, 4), "GFLOP/s": round(cfl / (cms * 1e-3) / 1e9, 1),

@@ -180,6 +180,19 @@ def bench_mixed(torch, P, steps: int, warmup: int):
         progress(f"mixed fused {dt} {hms:.4f} ms")
         del hp, xs, ys, hws
 
+    # Q0.15: the reference's fixed-point plans take powers of two only, so
+    # the lengths are rounded down to powers of two (at most 32768)
+    qlen = [min(32768, 1 << (n.bit_length() - 1)) for n in lengths]
+    qp = P.MixedPlan(qlen, P.PLFFT_FORWARD, "q15")
+    qx = torch.randint(-32768, 32768, (2 * sum(qlen),), dtype=torch.int16, device="cuda")
+    qy = torch.empty_like(qx)
+    qms = time_stream(torch, lambda s: qp.execute_async(qx, qy, stream=s), steps, warmup)
+    res["fused_q15_pow2_ms"] = round(qms, 4)
+    res["q15_pow2_points"] = int(sum(qlen))
+    res["q15_plan"] = qp.describe()
+    progress(f"mixed fused q15 (powers of two) {qms:.4f} ms")
+    del qp, qx, qy
+
     # plain: one plan per distinct length (creation not timed)
     t0 = time.perf_counter()
     plans = {}

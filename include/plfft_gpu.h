@@ -73,14 +73,16 @@ const char *plfft_last_error(void);
  * not otherwise overlap. Unnormalized, like every plfft transform. Lengths up
  * to 131072 are accepted for any n, and up to 262144 for n whose prime
  * factors are all at most 23. data_type: FP32, FP64, FP16 or BF16 (FP32
- * arithmetic for the 16-bit types).
+ * arithmetic for the 16-bit types), or Q15 / Q7 with power-of-two lengths up
+ * to 32768 (one launch per distinct length; other lengths fail, as the
+ * reference's fixed-point plans do).
  * ---------------------------------------------------------------------- */
 typedef struct plfft_mixed_plan_t plfft_mixed_plan_t;
 
 /* Returns PLFFT_NULL_ARGUMENT (null plan or n), PLFFT_INVALID_SIZE (count < 1,
  * a length < 1, a negative offset), PLFFT_INVALID_TRANSFORM_KIND (direction),
  * PLFFT_INVALID_DATA_TYPE, PLFFT_INVALID_PLANNER_RIGOR, or
- * PLFFT_PLAN_CREATION_FAILED (unsupported length or type, no device; see
+ * PLFFT_PLAN_CREATION_FAILED (unsupported length, no device; see
  * plfft_last_error()). */
 plfft_status_t plfft_mixed_plan_create(plfft_mixed_plan_t **plan, int64_t count,
                                        const int64_t *n, const int64_t *in_offset,

@@ -605,16 +605,14 @@ extern "C" plfft_status_t plfft_mixed_plan_create(plfft_mixed_plan_t **plan, int
       return PLFFT_INVALID_SIZE;
   }
   if (!valid_direction(direction)) return PLFFT_INVALID_TRANSFORM_KIND;
-  int prec;
+  int prec = plf::P_F32, fixed_bits = 0;
   switch ((int)data_type) {
     case PLFFT_DATA_TYPE_FP32: prec = plf::P_F32; break;
     case PLFFT_DATA_TYPE_FP64: prec = plf::P_F64; break;
     case PLFFT_DATA_TYPE_FP16: prec = plf::P_F16; break;
     case 5 /* PLFFT_DATA_TYPE_BF16 */: prec = plf::P_BF16; break;
-    case PLFFT_DATA_TYPE_Q7:
-    case PLFFT_DATA_TYPE_Q15:
-      plf::set_last_error("mixed batch: fixed-point storage is not supported");
-      return PLFFT_PLAN_CREATION_FAILED;
+    case PLFFT_DATA_TYPE_Q7: fixed_bits = 7; break;
+    case PLFFT_DATA_TYPE_Q15: fixed_bits = 15; break;
     default: return PLFFT_INVALID_DATA_TYPE;
   }
   if ((int)rigor < PLFFT_ESTIMATE || (int)rigor > PLFFT_EXHAUSTIVE)
@@ -632,7 +630,7 @@ extern "C" plfft_status_t plfft_mixed_plan_create(plfft_mixed_plan_t **plan, int
   std::string why;
   plf::set_planning_rigor(rigor);
   std::unique_ptr<plf::MixedPlan> mp =
-      plf::MixedPlan::build(prec, direction, count, n, in_offset, out_offset, &why);
+      plf::MixedPlan::build(prec, fixed_bits, direction, count, n, in_offset, out_offset, &why);
   plf::set_planning_rigor(0);
   if (!mp) {
     plf::set_last_error(why);
@@ -642,7 +640,7 @@ extern "C" plfft_status_t plfft_mixed_plan_create(plfft_mixed_plan_t **plan, int
   if (!pl) out_of_memory();
   pl->mp = std::move(mp);
   pl->device = dev;
-  pl->elem = 2 * (size_t)plf::prec_real_bytes(prec);
+  pl->elem = fixed_bits ? (fixed_bits == 15 ? 4 : 2) : 2 * (size_t)plf::prec_real_bytes(prec);
   plf::set_last_error("");
   *plan = pl;
   return PLFFT_OK;

@@ -55,6 +55,13 @@ class FixedC2CNode : public Node {
                int64_t od)
       : bits_(bits), n_(n), bwd_(bwd), batch_(batch), is_(is), id_(id), os_(os), od_(od) {}
 
+  // members at explicit offsets (complex elements), unit element stride
+  hipError_t set_offsets(const std::vector<int64_t> &in_off, const std::vector<int64_t> &out_off) {
+    hipError_t err = ioff_.upload(in_off.data(), in_off.size() * sizeof(int64_t));
+    if (err == hipSuccess) err = ooff_.upload(out_off.data(), out_off.size() * sizeof(int64_t));
+    return err;
+  }
+
   hipError_t init() {
     tier_ = -1;
     for (int t = 0; t < kNumFixedTiers; ++t)
@@ -108,7 +115,9 @@ class FixedC2CNode : public Node {
     a.ostride = os_;
     a.odist = od_;
     a.n = n_;
-    a.flags = bwd_ ? 1 : 0;
+    a.flags = (bwd_ ? 1 : 0) | (ioff_.p ? 2 : 0);
+    a.aux0 = ioff_.p;
+    a.aux1 = ooff_.p;
     const int64_t grid = (count + t.tpb - 1) / t.tpb;
     const size_t lds =
         (size_t)t.tpb * (size_t)(n_ + (n_ >> 5) + 1) * (bits_ == 15 ? 4 : 2);
@@ -129,7 +138,7 @@ class FixedC2CNode : public Node {
   bool bwd_;
   int64_t batch_, is_, id_, os_, od_;
   int tier_ = -1;
-  DevMem desc_, tab_;
+  DevMem desc_, tab_, ioff_, ooff_;
 };
 
 // R2C (forward) or C2R (backward) through a complex staging buffer.
@@ -230,6 +239,26 @@ std::unique_ptr<Node> build_fixed_node(const Problem &p, std::string *why) {
   }
   *why = std::string("fixed-point table setup failed: ") + hipGetErrorString(err);
   return nullptr;
+}
+
+// Fixed-point members of one power-of-two length at explicit offsets (the
+// groups of a fixed-point mixed-length batch).
+std::unique_ptr<Node> build_fixed_gather_node(int bits, int64_t n, bool bwd,
+                                              const std::vector<int64_t> &in_off,
+                                              const std::vector<int64_t> &out_off,
+                                              std::string *why) {
+  if (!pow2(n) || n > kFixedTiers[kNumFixedTiers - 1].max_n) {
+    *why = "fixed-point mixed batches need power-of-two lengths <= 32768";
+    return nullptr;
+  }
+  auto node = std::make_unique<FixedC2CNode>(bits, n, bwd, (int64_t)in_off.size(), 1, 0, 1, 0);
+  hipError_t err = node->init();
+  if (err == hipSuccess) err = node->set_offsets(in_off, out_off);
+  if (err != hipSuccess) {
+    *why = std::string("fixed-point table setup failed: ") + hipGetErrorString(err);
+    return nullptr;
+  }
+  return node;
 }
 
 }  // namespace plf

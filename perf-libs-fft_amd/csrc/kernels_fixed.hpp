@@ -216,7 +216,9 @@ struct FixedEngine {
 
 // Batched fixed-point C2C. KArgs: in/out (storage pairs), tw = 16 codelet
 // constants Q(W_16^k) followed by the stage twiddles, desc, count, strides
-// and distances in complex elements, n, flags bit0 = backward.
+// and distances in complex elements, n, flags bit0 = backward; flags bit1:
+// member offsets (complex elements) from the tables aux0 (input) and aux1
+// (output) instead of member * distance.
 template <int BITS, int TPT, int TPB, int PTS>
 __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   using P = QPair<BITS>;
@@ -226,8 +228,13 @@ __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   const bool valid = member < a.count;
   P *lds = reinterpret_cast<P *>(plf_dyn_lds) + tb * lds_padded<5>((int)a.n);
   const P *tab = static_cast<const P *>(a.tw);
-  const P *in = static_cast<const P *>(a.in) + (valid ? member * a.idist : 0);
-  P *out = static_cast<P *>(a.out) + (valid ? member * a.odist : 0);
+  const bool table = a.flags & 2;
+  const int64_t ioff = !valid ? 0 : table ? static_cast<const int64_t *>(a.aux0)[member]
+                                          : member * a.idist;
+  const int64_t ooff = !valid ? 0 : table ? static_cast<const int64_t *>(a.aux1)[member]
+                                          : member * a.odist;
+  const P *in = static_cast<const P *>(a.in) + ioff;
+  P *out = static_cast<P *>(a.out) + ooff;
   const bool bwd = a.flags & 1;
   const int64_t is = a.istride, os = a.ostride;
   FixedEngine<BITS, TPT, PTS>::run(

@@ -156,14 +156,40 @@ bool schedule_for(int64_t n, Schedule *s, std::string *why) {
 
 }  // namespace
 
-std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, const int64_t *n,
-                                            const int64_t *in_off, const int64_t *out_off,
-                                            std::string *why) {
+std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, int64_t count,
+                                            const int64_t *n, const int64_t *in_off,
+                                            const int64_t *out_off, std::string *why) {
   auto plan = std::unique_ptr<MixedPlan>(new MixedPlan);
   MixedPlan &P = *plan;
   P.prec_ = prec;
   P.bwd_ = dir > 0 ? 1 : 0;
   P.count_ = count;
+  if (fixed_bits) {
+    // The reference's fixed-point plans are power-of-two only
+    // (src/planner/planner.cpp:38-47): one gather launch per length.
+    std::map<int64_t, std::pair<std::vector<int64_t>, std::vector<int64_t>>> groups;
+    int64_t ipos = 0, opos = 0;
+    for (int64_t i = 0; i < count; ++i) {
+      const int64_t io = in_off ? in_off[i] : ipos, oo = out_off ? out_off[i] : opos;
+      ipos += n[i];
+      opos += n[i];
+      P.in_span_ = std::max(P.in_span_, io + n[i]);
+      P.out_span_ = std::max(P.out_span_, oo + n[i]);
+      groups[n[i]].first.push_back(io);
+      groups[n[i]].second.push_back(oo);
+    }
+    for (const auto &g : groups) {
+      std::unique_ptr<Node> node = build_fixed_gather_node(fixed_bits, g.first, dir > 0,
+                                                           g.second.first, g.second.second, why);
+      if (!node) return nullptr;
+      P.fixed_groups_.push_back(std::move(node));
+    }
+    std::ostringstream o;
+    o << "(mixed_fixed_q" << fixed_bits << " count=" << count << " lengths=" << groups.size()
+      << " launches=" << groups.size() << ")";
+    P.summary_ = o.str();
+    return plan;
+  }
   const int csize = 2 * prec_compute_bytes(prec);
 
   std::map<int64_t, Schedule> sched;
@@ -374,6 +400,13 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int dir, int64_t count, co
 }
 
 hipError_t MixedPlan::enqueue(const void *in, void *out, void *ws, hipStream_t s) const {
+  if (!fixed_groups_.empty()) {
+    for (const auto &g : fixed_groups_) {
+      const hipError_t err = g->enqueue(in, out, nullptr, s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
   MixedArgs a{};
   a.in = in;
   a.out = out;

@@ -16,9 +16,11 @@ class MixedPlan {
   // Member i transforms n[i] complex elements read at in_off[i] and written
   // at out_off[i] (complex elements of the storage type). A null offset
   // array packs the members back to back. dir: -1 forward, +1 backward.
-  static std::unique_ptr<MixedPlan> build(int prec, int dir, int64_t count, const int64_t *n,
-                                          const int64_t *in_off, const int64_t *out_off,
-                                          std::string *why);
+  // fixed_bits 15 / 7: Q0.15 / Q0.7 storage (power-of-two n <= 32768, one
+  // launch per distinct length); 0: floating point (prec).
+  static std::unique_ptr<MixedPlan> build(int prec, int fixed_bits, int dir, int64_t count,
+                                          const int64_t *n, const int64_t *in_off,
+                                          const int64_t *out_off, std::string *why);
 
   // Launch the kernel on `s` once per non-empty dependency level (at most
   // three launches, nothing else). `ws` holds workspace_bytes() of device
@@ -42,6 +44,7 @@ class MixedPlan {
   int64_t level_first_[3] = {0, 0, 0}, level_count_[3] = {0, 0, 0};
   size_t lds_ = 0, data_bytes_ = 0;
   DevMem xf_, tasks_, descs_, desc_tw_, tw_, aux_;
+  std::vector<std::unique_ptr<Node>> fixed_groups_;  // fixed point: one per length
   std::string summary_;
 };
 

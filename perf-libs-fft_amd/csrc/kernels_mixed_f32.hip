@@ -1,4 +1,4 @@
-// kernels_mixed_f32.hip -- persistent mixed-length kernel, FP32 storage.
+// kernels_mixed_f32.hip -- mixed-length batch kernel, FP32 storage.
 #include "kernels_mixed_impl.hpp"
 
 namespace plf {

@@ -1,4 +1,4 @@
-// kernels_mixed_f64.hip -- persistent mixed-length kernel, FP64 storage.
+// kernels_mixed_f64.hip -- mixed-length batch kernel, FP64 storage.
 #include "kernels_mixed_impl.hpp"
 
 namespace plf {

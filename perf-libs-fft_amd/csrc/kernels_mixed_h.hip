@@ -1,4 +1,4 @@
-// kernels_mixed_h.hip -- persistent mixed-length kernel, FP16 and BF16
+// kernels_mixed_h.hip -- mixed-length batch kernel, FP16 and BF16
 // storage (FP32 arithmetic), plus the precision dispatch.
 #include "kernels_mixed_impl.hpp"
 #include "registry.hpp"

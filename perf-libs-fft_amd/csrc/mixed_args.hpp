@@ -1,4 +1,4 @@
-// mixed_args.hpp -- host/device records of the mixed-length persistent launch.
+// mixed_args.hpp -- host/device records of the mixed-length batch kernel.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -54,9 +54,9 @@ struct MixedArgs {
   int bwd;                 // 1: backward transform (exponent sign +)
 };
 
-// Launch the persistent kernel for storage precision `prec` (Prec values).
+// Launch the mixed-batch kernel for storage precision `prec` (Prec values).
 hipError_t launch_mixed(int prec, const MixedArgs &a, int grid, size_t lds, hipStream_t s);
-// Resident blocks per CU of the persistent kernel with `lds` bytes of LDS.
+// Resident blocks per CU of the mixed-batch kernel with `lds` bytes of LDS.
 int mixed_blocks_per_cu(int prec, size_t lds);
 
 }  // namespace plf

@@ -1,4 +1,4 @@
-// mixed_plan.cpp -- one persistent launch for a batch of mixed lengths.
+// mixed_plan.cpp -- a batch of mixed lengths in at most three launches.
 //
 // The reference executes a batch as one plan per length (src/plfft.cpp:
 // 283-330 builds a plan for a single n; a caller with many lengths loops over

@@ -1,5 +1,5 @@
-// mixed_plan.hpp -- plan of one persistent launch over a batch of C2C
-// transforms of different lengths.
+// mixed_plan.hpp -- plan of a batch of C2C transforms of different lengths,
+// run in at most three launches (one per dependency level).
 #pragma once
 
 #include <memory>

@@ -1,4 +1,4 @@
-"""Mixed-length batches in one persistent launch (plfft_mixed_plan_* of
+"""Mixed-length batches (plfft_mixed_plan_* of
 include/plfft_gpu.h) against the FP64 oracle, member by member.
 
 The reference has no mixed-length object; its caller runs one plan per

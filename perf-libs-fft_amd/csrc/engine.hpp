@@ -119,12 +119,15 @@ struct StaticEngine {
 
   // Stage 0 takes its inputs from registers: vin[k * R0 + m] holds element
   // b + m * (N / R0) of butterfly b = t + k * TPT (zero where b >= N / R0).
-  // Lets a caller issue the next member's loads before this member's stages.
-  template <class Store>
+  // Lets a caller issue the next member's loads before this member's stages,
+  // or compute stage-0 inputs with knowledge of the butterfly they feed.
+  // With TO_LDS the results stay in LDS as for run<true>.
+  template <bool TO_LDS = false, class Store>
   static __device__ __forceinline__ void run_regs(const cpx<C> *vin, cpx<C> *lds,
-                                                  const cpx<C> *tw, int t, Store &&store) {
+                                                  const cpx<C> *tw, int t, Store &&store,
+                                                  bool valid = true) {
     auto none = [](int) { return cpx<C>{C(0), C(0)}; };
-    stage<0, false, false, decltype(none), Store, true>(lds, tw, t, true, none, store, vin);
+    stage<0, TO_LDS, false, decltype(none), Store, true>(lds, tw, t, valid, none, store, vin);
   }
 };
 
@@ -245,6 +248,22 @@ struct StaticEngineSplit {
       for (int m = 0; m < T::R; ++m)
         v[k][m] = (T::act(b) && valid) ? load(b + m * T::NB) : cpx<C>{C(0), C(0)};
     }
+    from<0>(v, reinterpret_cast<C *>(lds), tw, t, valid, store);
+  }
+
+  // Stage 0 takes its inputs from registers, laid out as for
+  // StaticEngine::run_regs.
+  template <bool TO_LDS = false, class Store>
+  static __device__ __forceinline__ void run_regs(const cpx<C> *vin, cpx<C> *lds,
+                                                  const cpx<C> *tw, int t, Store &&store,
+                                                  bool valid = true) {
+    static_assert(!TO_LDS, "split exchange: global output only");
+    using T = St<0>;
+    cpx<C> v[T::KB][T::R];
+#pragma unroll
+    for (int k = 0; k < T::KB; ++k)
+#pragma unroll
+      for (int m = 0; m < T::R; ++m) v[k][m] = vin[k * T::R + m];
     from<0>(v, reinterpret_cast<C *>(lds), tw, t, valid, store);
   }
 };

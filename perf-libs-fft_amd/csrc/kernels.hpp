@@ -26,6 +26,8 @@ struct SEngX {
   using Ch = Chain<Rs...>;
   using E = StaticEngine<C, TPT, PAD, Ch, PAIR_>;
   static constexpr int NFIX = Ch::N;
+  // stage-0 inputs may be handed over in registers (E::run_regs)
+  static constexpr bool REGS_IN = !PAIR_;
   static constexpr bool USES_LDS = E::USES_LDS;
   static __host__ __device__ constexpr int lds_per(int64_t) { return lds_padded<PAD>(Ch::N); }
   template <bool TO_LDS, bool LDS_IN = false, class Load, class Store>
@@ -47,6 +49,7 @@ struct SEngS {
   using Ch = Chain<Rs...>;
   using E = StaticEngineSplit<C, TPT, PAD, Ch>;
   static constexpr int NFIX = Ch::N;
+  static constexpr bool REGS_IN = true;
   static constexpr bool USES_LDS = E::USES_LDS;
   static __host__ __device__ constexpr int lds_per(int64_t) {
     return (lds_padded<PAD>(Ch::N) + 1) / 2;
@@ -65,6 +68,7 @@ struct DEng {
   static constexpr int PTS = PTS_;
   static constexpr int PAD = 5;
   static constexpr int NFIX = 0;
+  static constexpr bool REGS_IN = false;
   static constexpr bool USES_LDS = true;
   static constexpr bool BIGP = TPT_ <= 256 && TPB_ <= 16 && TPT_ * TPB_ <= 256;
   using E = DynamicEngine<C, TPT, PTS, BIGP>;

@@ -29,6 +29,12 @@ void register_kernels_f16(std::vector<KEntry> &r) {
   PLF_DYNAMIC(k_row, KK_ROW, S, P,   64,  16, 16);
   PLF_DYNAMIC(k_col, KK_COL, S, P,  256,   4, 16);
   PLF_DYNAMIC(k_row, KK_ROW, S, P,  256,   4, 16);
+  // unit stride: non-temporal loads and stores, no stride arithmetic
+  //              TPT  TPB PAD PAIR   radix chain
+  PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
+  PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
+  PLF_UNIT(S, P, 256,   1, 4, false, 16, 16, 16);
+  PLF_UNIT(S, P, 1024,  1, 4, false, 16, 16, 16, 4);
 }
 
 }  // namespace plf

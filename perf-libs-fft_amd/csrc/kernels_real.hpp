@@ -132,25 +132,50 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
       y[(int64_t)(2 * e + 1) * os] = ST::str(v.y);
     }
   };
-  Eng::template run<false>(
-      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
-      [&](int e) {
-        cpx<C> A = ST::ld(X[(int64_t)e * is]);
-        cpx<C> B = ST::ld(X[(int64_t)(m - e) * is]);
-        if (e == 0) {
-          A.y = C(0);
-          B.y = C(0);
-        } else {
-          A = {hw * A.x, hw * A.y};
-          B = {hw * B.x, hw * B.y};
-        }
-        const cpx<C> s = {A.x + B.x, A.y - B.y};   // A + conj B
-        const cpx<C> d = {A.x - B.x, A.y + B.y};   // A - conj B
-        const cpx<C> wd = cconj_mul(d, W[e]);      // W^{-e} (A - conj B)
-        const cpx<C> zp = {s.x - wd.y, s.y + wd.x};  // s + i wd
-        return cswap(zp);                           // inverse DFT via swap
-      },
-      store);
+  // Z'[e] for stage 0, w = W_n^e
+  auto pre = [&](int e, cpx<C> w) {
+    cpx<C> A = ST::ld(X[(int64_t)e * is]);
+    cpx<C> B = ST::ld(X[(int64_t)(m - e) * is]);
+    if (e == 0) {
+      A.y = C(0);
+      B.y = C(0);
+    } else {
+      A = {hw * A.x, hw * A.y};
+      B = {hw * B.x, hw * B.y};
+    }
+    const cpx<C> s = {A.x + B.x, A.y - B.y};     // A + conj B
+    const cpx<C> d = {A.x - B.x, A.y + B.y};     // A - conj B
+    const cpx<C> wd = cconj_mul(d, w);           // W^{-e} (A - conj B)
+    const cpx<C> zp = {s.x - wd.y, s.y + wd.x};  // s + i wd
+    return cswap(zp);                            // inverse DFT via swap
+  };
+  constexpr int NF = Eng::NFIX;
+  if constexpr (NF > 0 && Eng::REGS_IN) {
+    // compile-time length: thread t feeds the stage-0 butterflies
+    // b = t + k TPT with elements e = b + j NB (NB = m / R0), and
+    // W_n^e = W_n^b W_n^{j NB}: one per-thread twiddle load and R0 loads at
+    // block-uniform addresses instead of one per-lane load per element
+    using Ch = typename Eng::Ch;
+    constexpr int R0 = Ch::radix(0);
+    constexpr int NB = NF / R0;
+    constexpr int KB = (NB + TPT - 1) / TPT;
+    cpx<C> vin[KB * R0];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int b = t + k * TPT;
+      const bool act = valid && ((NB % TPT == 0) || (b < NB));
+      const cpx<C> wb = act ? W[b] : cpx<C>{C(1), C(0)};
+#pragma unroll
+      for (int j = 0; j < R0; ++j)
+        vin[k * R0 + j] =
+            act ? pre(b + j * NB, j == 0 ? wb : cmul(wb, W[j * NB])) : cpx<C>{C(0), C(0)};
+    }
+    Eng::E::run_regs(vin, lds, static_cast<const cpx<C> *>(a.tw), t, store, valid);
+  } else {
+    Eng::template run<false>(
+        a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+        [&](int e) { return pre(e, W[e]); }, store);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -242,6 +267,30 @@ __device__ __forceinline__ typename ST::C dct2_src(const typename ST::Rs *x, int
   return dst ? -v : v;
 }
 
+// Thread t's share of for (o = 0; o < len; ++o): first loads every value
+// (ld(o)), then consumes them (use(o, v)). With a compile-time length
+// (LEN > 0) the loop is unrolled, so all of the thread's loads are in flight
+// together.
+template <int TPT, int LEN, class V, class Ld, class Use>
+__device__ __forceinline__ void gather_then(int t, int len, Ld &&ld, Use &&use) {
+  if constexpr (LEN > 0) {
+    constexpr int IT = (LEN + TPT - 1) / TPT;
+    V v[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int o = t + i * TPT;
+      if (LEN % TPT == 0 || o < LEN) v[i] = ld(o);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int o = t + i * TPT;
+      if (LEN % TPT == 0 || o < LEN) use(o, v[i]);
+    }
+  } else {
+    for (int o = t; o < len; o += TPT) use(o, ld(o));
+  }
+}
+
 template <typename S, bool DST, int TPB, class Eng>
 __global__ void __launch_bounds__(Eng::TPT *TPB) k_dct2_even(KArgs a) {
   using ST = StorageT<S>;
@@ -258,12 +307,24 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_dct2_even(KArgs a) {
   Rs *y = static_cast<Rs *>(a.out) + (valid ? tr * a.odist : 0);
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(m);
   const int64_t is = a.istride, os = a.ostride;
-  Eng::template run<true>(
+  // Coalesced loads of the member's reals, scattered into LDS as the packed
+  // sequence z_e = u_{2e} + i u_{2e+1}: y-index o holds u_{o/2} (o even) or
+  // u_{n-(o+1)/2} (o odd, negated for DST).
+  constexpr int NF = Eng::NFIX;
+  const int nn = NF > 0 ? 2 * NF : n;
+  if (valid) {
+    C *lr = reinterpret_cast<C *>(lds);
+#pragma unroll 8
+    for (int o = t; o < nn; o += TPT) {
+      const C v = ST::ldr(x[(int64_t)o * is]);
+      const int i = (o & 1) ? n - (o + 1) / 2 : o / 2;
+      lr[2 * lds_pad<Eng::PAD>(i >> 1) + (i & 1)] = (DST && (o & 1)) ? -v : v;
+    }
+  }
+  __syncthreads();
+  Eng::template run<true, true>(
       a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
-      [&](int e) {
-        return cpx<C>{dct2_src<ST>(x, 2 * e, n, is, DST), dct2_src<ST>(x, 2 * e + 1, n, is, DST)};
-      },
-      [&](int, cpx<C>) {});
+      [&](int) { return cpx<C>{C(0), C(0)}; }, [&](int, cpx<C>) {});
   if (!valid) return;
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
   const cpx<C> *w = static_cast<const cpx<C> *>(a.aux1);
@@ -367,31 +428,78 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_dct3_even(KArgs a) {
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
   const cpx<C> *w = static_cast<const cpx<C> *>(a.aux1);
   const C s0 = (a.flags & 2) ? C(2) : C(1);
-  auto X = [&](int k) {  // (u_k - i u_{n-k}) * conj(w_k), k in [0, m]
-    const C ur = dct3_in<ST, DST>(x, k, n, is, s0);
-    const C ui = k == 0 ? C(0) : -dct3_in<ST, DST>(x, n - k, n, is, s0);
-    return cconj_mul(cpx<C>{ur, ui}, w[k]);
+  // stage-0 input e from X(e) and X(m-e), w = W_n^e
+  auto z = [&](cpx<C> A, cpx<C> B, cpx<C> w) {
+    const cpx<C> sm = {A.x + B.x, A.y - B.y};
+    const cpx<C> d = {A.x - B.x, A.y + B.y};
+    const cpx<C> wd = cconj_mul(d, w);
+    return cswap(cpx<C>{sm.x - wd.y, sm.y + wd.x});
   };
-  Eng::template run<false>(
-      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
-      [&](int e) {
-        const cpx<C> A = X(e);
-        const cpx<C> B = X(m - e);
-        const cpx<C> sm = {A.x + B.x, A.y - B.y};
-        const cpx<C> d = {A.x - B.x, A.y + B.y};
-        const cpx<C> wd = cconj_mul(d, W[e]);
-        return cswap(cpx<C>{sm.x - wd.y, sm.y + wd.x});
+  constexpr int NF = Eng::NFIX;
+  if constexpr (NF > 0 && Eng::REGS_IN) {
+    // compile-time length: the n reals u are staged in LDS with coalesced
+    // loads; thread t then builds the inputs of its stage-0 butterflies
+    // b = t + k TPT, elements e = b + j NB (NB = m / R0), with
+    // W_n^e = W_n^b W_n^{j NB}, w_e = w_b w_{j NB} and w_{m-e} = w_m conj(w_e)
+    C *lu = reinterpret_cast<C *>(lds);
+    if (valid) {
+      gather_then<TPT, 2 * NF, C>(
+          t, 2 * NF, [&](int k) { return dct3_in<ST, DST>(x, k, n, is, s0); },
+          [&](int k, C v) { lu[k] = v; });
+    }
+    __syncthreads();
+    using Ch = typename Eng::Ch;
+    constexpr int R0 = Ch::radix(0);
+    constexpr int NB = NF / R0;
+    constexpr int KB = (NB + TPT - 1) / TPT;
+    const cpx<C> wm = w[NF];
+    cpx<C> vin[KB * R0];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      const int b = t + k * TPT;
+      const bool act = valid && ((NB % TPT == 0) || (b < NB));
+      const cpx<C> Wb = act ? W[b] : cpx<C>{C(1), C(0)};
+      const cpx<C> wb = act ? w[b] : cpx<C>{C(1), C(0)};
+#pragma unroll
+      for (int j = 0; j < R0; ++j) {
+        const int e = b + j * NB;
+        const cpx<C> We = j == 0 ? Wb : cmul(Wb, W[j * NB]);
+        const cpx<C> we = j == 0 ? wb : cmul(wb, w[j * NB]);
+        const cpx<C> wme = cconj_mul(wm, we);
+        if (act) {
+          const cpx<C> A = cconj_mul(cpx<C>{lu[e], e == 0 ? C(0) : -lu[n - e]}, we);
+          const cpx<C> B = cconj_mul(cpx<C>{lu[NF - e], -lu[NF + e]}, wme);
+          vin[k * R0 + j] = z(A, B, We);
+        } else {
+          vin[k * R0 + j] = cpx<C>{C(0), C(0)};
+        }
+      }
+    }
+    __syncthreads();  // u is read; stage 0 overwrites the LDS
+    Eng::E::template run_regs<true>(vin, lds, static_cast<const cpx<C> *>(a.tw), t,
+                                    [&](int, cpx<C>) {}, valid);
+  } else {
+    auto X = [&](int k) {  // (u_k - i u_{n-k}) * conj(w_k), k in [0, m]
+      const C ur = dct3_in<ST, DST>(x, k, n, is, s0);
+      const C ui = k == 0 ? C(0) : -dct3_in<ST, DST>(x, n - k, n, is, s0);
+      return cconj_mul(cpx<C>{ur, ui}, w[k]);
+    };
+    Eng::template run<true>(
+        a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+        [&](int e) { return z(X(e), X(m - e), W[e]); }, [&](int, cpx<C>) {});
+  }
+  if (!valid) return;
+  // swap(lds[e]) = v_{2e} + i v_{2e+1}; coalesced stores y[o] = v_i with
+  // i = o/2 (o even) or n - (o+1)/2 (o odd, negated for DST)
+  const int nn = NF > 0 ? 2 * NF : n;
+  const C *lr = reinterpret_cast<const C *>(lds);
+  gather_then<TPT, 2 * NF, C>(
+      t, nn,
+      [&](int o) {
+        const int i = (o & 1) ? n - (o + 1) / 2 : o / 2;
+        return lr[2 * lds_pad<Eng::PAD>(i >> 1) + 1 - (i & 1)];
       },
-      [&](int e, cpx<C> v) {
-        v = cswap(v);
-        const int i0 = 2 * e, i1 = 2 * e + 1;
-        const int o0 = 2 * i0 < n ? 2 * i0 : 2 * (n - i0) - 1;
-        const int o1 = 2 * i1 < n ? 2 * i1 : 2 * (n - i1) - 1;
-        const C v0 = (DST && (o0 & 1)) ? -v.x : v.x;
-        const C v1 = (DST && (o1 & 1)) ? -v.y : v.y;
-        y[(int64_t)o0 * os] = ST::str(v0);
-        y[(int64_t)o1 * os] = ST::str(v1);
-      });
+      [&](int o, C v) { y[(int64_t)o * os] = ST::str((DST && (o & 1)) ? -v : v); });
 }
 
 template <typename S, bool DST, int TPB, class Eng>

@@ -114,6 +114,16 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
           &::plf::KERNEL<S, false, TPB, ::plf::DEng<typename ::plf::StorageT<S>::C, TPT, PTS, TPB>>, \
           TPT * TPB>)
 
+// Single-direction kind on the split (real, then imaginary) LDS exchange.
+#define PLF_STATIC1_SPLIT(KERNEL, KK, S, PREC, TPT, TPB, PAD, ...)                         \
+  do {                                                                                    \
+    using EngT = ::plf::SEngS<typename ::plf::StorageT<S>::C, TPT, PAD, __VA_ARGS__>;      \
+    ::plf::add_entry1(r, KK, PREC, ::plf::Chain<__VA_ARGS__>::N, {__VA_ARGS__}, TPT, TPB, 0, \
+                      true, &::plf::launch_k<&::plf::KERNEL<S, false, TPB, EngT>, TPT * TPB>); \
+    r.back().pad = PAD;                                                                   \
+    r.back().lds_scalars = 1;                                                             \
+  } while (0)
+
 // Unit-stride streaming C2C with an explicit LDS padding shift and
 // butterfly-pairing choice.
 #define PLF_UNIT(S, PREC, TPT, TPB, PAD, PAIR, ...)                                       \

@@ -52,10 +52,9 @@ std::string KEntry::name() const {
     o << "_dyn";
   }
   o << "_t" << tpt << "_b" << tpb;
-  if (kind == KK_C2C_UNIT) {
-    o << "_p" << pad << (pts ? "_pair" : "") << (lds_scalars == 1 ? "_split" : "");
-    if (persist) o << "_pipe" << persist;
-  }
+  if (kind == KK_C2C_UNIT) o << "_p" << pad << (pts ? "_pair" : "");
+  if (lds_scalars == 1) o << "_split";
+  if (persist) o << "_pipe" << persist;
   return o.str();
 }
 

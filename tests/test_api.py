@@ -31,6 +31,24 @@ def test_library_exports_every_declared_symbol(lib):
         assert getattr(lib, name) is not None
 
 
+def test_every_host_kernel_has_device_code():
+    """Each kernel the host side can launch (a __device_stub__) has a gfx950
+    kernel descriptor (<name>.kd) in the embedded code object; a stale or
+    half-rebuilt object would otherwise fail only at launch time."""
+    data = open(P.LIB_PATH, "rb").read()
+    prefix = b"__device_stub__"
+    kds = set(re.findall(rb"(_ZN3plf[A-Za-z0-9_]+)\.kd\x00", data))
+    stubs = set(re.findall(rb"_ZN3plf(\d+)__device_stub__([A-Za-z0-9_]+)\x00", data))
+    assert len(stubs) > 100
+    missing = []
+    for length, body in stubs:
+        k = int(length) - len(prefix)  # identifier length of the kernel's own name
+        kernel = b"_ZN3plf" + str(k).encode() + body
+        if kernel not in kds:
+            missing.append(kernel.decode())
+    assert not missing, missing[:5]
+
+
 def test_reference_abi_function_count():
     text = open(os.path.join(ROOT, "include", "plfft.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)

@@ -111,8 +111,9 @@ def progress(msg: str) -> None:
 def bench_mixed(torch, P, steps: int, warmup: int):
     """The mixed-length batch: 10,000 C2C members, N log-uniform in
     [16, 65536], 10 % moved to a prime, packed back to back, forward,
-    out-of-place. Fused = one persistent launch (plus a counter fill);
-    plain = one ordinary plan per member, executed one after another
+    out-of-place. Fused = one mixed plan, at most three launches (one per
+    dependency level of its tasks); plain = one ordinary plan per member,
+    executed one after another
     (eagerly, and captured in a graph to take host launch cost out)."""
     import numpy as np
 

@@ -249,7 +249,7 @@ def main() -> int:
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration)")
     ap.add_argument("--only", default=None,
-                    help="run one case kind:n:dtype[:bwd][:inplace] (profiling)")
+                    help="run one case kind:n:dtype[:bwd][:inplace][:r2r kind] (profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -300,8 +300,9 @@ def main() -> int:
         kind, cn, prec = parts[0], int(parts[1]), parts[2]
         cdir = 1 if "bwd" in parts[3:] else -1
         inplace = "inplace" in parts[3:]
+        r2r = next((q for q in parts[3:] if q in P.R2R_KINDS), 0)
         cb = total // cn if prec != "f64" else total // 2 // cn
-        cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace)
+        cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace, r2r=r2r)
         cms = time_plan(torch, cplan, x, x if inplace else y, args.steps, args.warmup)
         print(json.dumps({"case": args.only, "batch": cb, "ms": round(cms, 5),
                           "plan": cplan.describe()}))

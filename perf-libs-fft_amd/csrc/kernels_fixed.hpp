@@ -18,6 +18,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fft_device.hpp"
 #include "kernel_args.hpp"
 
@@ -38,67 +40,131 @@ struct qc {
   int re, im;
 };
 
+// Scalar form: each component widened to int32, every result clamped.
 template <int BITS>
 struct QOps {
   static constexpr int HI = (1 << BITS) - 1, LO = -(1 << BITS);
   using P = QPair<BITS>;
+  using V = qc;
   static __device__ __forceinline__ int sat(int v) { return v > HI ? HI : (v < LO ? LO : v); }
-  static __device__ __forceinline__ qc add(qc a, qc b) { return {sat(a.re + b.re), sat(a.im + b.im)}; }
-  static __device__ __forceinline__ qc sub(qc a, qc b) { return {sat(a.re - b.re), sat(a.im - b.im)}; }
-  // x * (-i) = (im, -re); the negation saturates (-LO > HI)
-  static __device__ __forceinline__ qc negi(qc a) { return {a.im, sat(-a.re)}; }
-  // srshr: (v + 2^(s-1)) >> s, arithmetic (floor)
-  template <int S>
-  static __device__ __forceinline__ int rshr(int v) {
-    if constexpr (S == 0) return v;
-    else return (v + (1 << (S - 1))) >> S;
-  }
-  // complex product, one rounding per component
-  static __device__ __forceinline__ qc cmul(qc x, qc w) {
-    constexpr int H = 1 << (BITS - 1);
-    return {sat((x.re * w.re - x.im * w.im + H) >> BITS),
-            sat((x.re * w.im + x.im * w.re + H) >> BITS)};
-  }
-  static __device__ __forceinline__ qc unpack(P p) { return {(int)p.re, (int)p.im}; }
-  static __device__ __forceinline__ P pack(qc v) {
+  static __device__ __forceinline__ V zero() { return {0, 0}; }
+  static __device__ __forceinline__ V in(P p) { return {(int)p.re, (int)p.im}; }
+  static __device__ __forceinline__ P out(V v) {
     P p;
     p.re = (decltype(p.re))v.re;
     p.im = (decltype(p.im))v.im;
     return p;
   }
+  static __device__ __forceinline__ V swap(V v) { return {v.im, v.re}; }
+  static __device__ __forceinline__ V add(V a, V b) { return {sat(a.re + b.re), sat(a.im + b.im)}; }
+  static __device__ __forceinline__ V sub(V a, V b) { return {sat(a.re - b.re), sat(a.im - b.im)}; }
+  // x * (-i) = (im, -re); the negation saturates (-LO > HI)
+  static __device__ __forceinline__ V negi(V a) { return {a.im, sat(-a.re)}; }
+  // srshr: (v + 2^(s-1)) >> s, arithmetic (floor), per component
+  template <int S>
+  static __device__ __forceinline__ V shr(V v) {
+    if constexpr (S == 0) return v;
+    else return {(v.re + (1 << (S - 1))) >> S, (v.im + (1 << (S - 1))) >> S};
+  }
+  // complex product, one rounding per component
+  static __device__ __forceinline__ V cmul(V x, P wp) {
+    constexpr int H = 1 << (BITS - 1);
+    const V w = in(wp);
+    return {sat((x.re * w.re - x.im * w.im + H) >> BITS),
+            sat((x.re * w.im + x.im * w.re + H) >> BITS)};
+  }
+};
 
-  static __device__ __forceinline__ void dft4(qc *a) {
-    const qc t0 = add(a[0], a[2]), t1 = sub(a[0], a[2]);
-    const qc t2 = add(a[1], a[3]), t3 = negi(sub(a[1], a[3]));
-    a[0] = add(t0, t2);
-    a[2] = sub(t0, t2);
-    a[1] = add(t1, t3);
-    a[3] = sub(t1, t3);
+// Packed Q0.15 form: (re, im) in one 32-bit register as two int16 lanes,
+// the same results bit for bit.
+//   add/sub   v_pk_add_i16 / v_pk_sub_i16 with clamp (saturating)
+//   shr<S>    floor(v / 2^S) + bit S-1 of v == floor((v + 2^(S-1)) / 2^S)
+//   cmul      two v_dot2 (exact int32 sums of int16 products) + rounding
+//             shift + saturating pack. Every product operand x has been
+//             shifted right by at least one bit first (|x| <= 2^14), so the
+//             16-bit negation of x.im is exact.
+typedef short q15x2 __attribute__((ext_vector_type(2)));
+
+struct QOps15P {
+  using P = QPair<15>;
+  using V = q15x2;
+  static __device__ __forceinline__ V zero() { return V{0, 0}; }
+  static __device__ __forceinline__ V in(P p) { return __builtin_bit_cast(V, p); }
+  static __device__ __forceinline__ P out(V v) { return __builtin_bit_cast(P, v); }
+  static __device__ __forceinline__ V swap(V v) { return __builtin_shufflevector(v, v, 1, 0); }
+  static __device__ __forceinline__ V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
+  static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
+  static __device__ __forceinline__ V negi(V a) {
+    const V n = __builtin_elementwise_sub_sat(V{0, 0}, a);
+    return __builtin_shufflevector(a, n, 1, 2);  // (a.im, sat(-a.re))
+  }
+  template <int S>
+  static __device__ __forceinline__ V shr(V v) {
+    if constexpr (S == 0) {
+      return v;
+    } else {
+      const V q = v >> (V)S;
+      const V r = (v >> (V)(S - 1)) & (V)1;
+      return q + r;
+    }
+  }
+  static __device__ __forceinline__ V cmul(V x, P wp) {
+    const V w = in(wp);
+    const V xn = {x.x, (short)(-x.y)};
+    const int re = __builtin_amdgcn_sdot2(xn, w, 1 << 14, false) >> 15;
+    const int im = __builtin_amdgcn_sdot2(x, swap(w), 1 << 14, false) >> 15;
+    const int lo = re > 32767 ? 32767 : (re < -32768 ? -32768 : re);
+    const int hi = im > 32767 ? 32767 : (im < -32768 ? -32768 : im);
+    return V{(short)lo, (short)hi};
+  }
+};
+
+template <int BITS>
+struct QSel {
+  using type = QOps<BITS>;
+};
+template <>
+struct QSel<15> {
+  using type = QOps15P;
+};
+
+// Radix-2/4/8/16 butterflies on shifted inputs over either form;
+// w16[k] = Q(W_16^k).
+template <class O>
+struct QBfly {
+  using V = typename O::V;
+  using P = typename O::P;
+  static __device__ __forceinline__ void dft4(V *a) {
+    const V t0 = O::add(a[0], a[2]), t1 = O::sub(a[0], a[2]);
+    const V t2 = O::add(a[1], a[3]), t3 = O::negi(O::sub(a[1], a[3]));
+    a[0] = O::add(t0, t2);
+    a[2] = O::sub(t0, t2);
+    a[1] = O::add(t1, t3);
+    a[3] = O::sub(t1, t3);
   }
 
-  // radix-R butterfly on shifted inputs; w16[k] = Q(W_16^k)
   template <int R>
-  static __device__ __forceinline__ void dft(qc *a, const P *w16) {
+  static __device__ __forceinline__ void dft(V *a, const P *w16) {
     if constexpr (R == 2) {
-      const qc x = add(a[0], a[1]), y = sub(a[0], a[1]);
+      const V x = O::add(a[0], a[1]), y = O::sub(a[0], a[1]);
       a[0] = x;
       a[1] = y;
     } else if constexpr (R == 4) {
       dft4(a);
     } else if constexpr (R == 8) {
-      qc e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
+      V e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
       dft4(e);
       dft4(o);
-      o[1] = cmul(o[1], unpack(w16[2]));
-      o[2] = negi(o[2]);
-      o[3] = cmul(o[3], unpack(w16[6]));
+      o[1] = O::cmul(o[1], w16[2]);
+      o[2] = O::negi(o[2]);
+      o[3] = O::cmul(o[3], w16[6]);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        a[k] = add(e[k], o[k]);
-        a[k + 4] = sub(e[k], o[k]);
+        a[k] = O::add(e[k], o[k]);
+        a[k + 4] = O::sub(e[k], o[k]);
       }
     } else if constexpr (R == 16) {
-      qc f[4][4];
+      V f[4][4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
 #pragma unroll
@@ -110,11 +176,11 @@ struct QOps {
 #pragma unroll
         for (int k1 = 1; k1 < 4; ++k1) {
           const int e = j * k1;
-          f[j][k1] = e == 4 ? negi(f[j][k1]) : cmul(f[j][k1], unpack(w16[e]));
+          f[j][k1] = e == 4 ? O::negi(f[j][k1]) : O::cmul(f[j][k1], w16[e]);
         }
 #pragma unroll
       for (int k1 = 0; k1 < 4; ++k1) {
-        qc g[4] = {f[0][k1], f[1][k1], f[2][k1], f[3][k1]};
+        V g[4] = {f[0][k1], f[1][k1], f[2][k1], f[3][k1]};
         dft4(g);
 #pragma unroll
         for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = g[k2];
@@ -127,7 +193,8 @@ struct QOps {
 // pair type. Stage 0 reads global memory, the last stage writes it.
 template <int BITS, int TPT, int PTS>
 struct FixedEngine {
-  using O = QOps<BITS>;
+  using O = typename QSel<BITS>::type;
+  using V = typename O::V;
   using P = QPair<BITS>;
   static constexpr int PAD = 5;
 
@@ -141,36 +208,37 @@ struct FixedEngine {
     asm volatile("" : "+v"(t));  // keep per-stage addressing out of the stage loop
     const int n = d->n;
     const int NB = n / R;
-    const int L = d->L[s];
+    const int L = d->L[s];         // a power of two: radices 2, 4, 8, 16 only
+    const int lgL = __builtin_ctz(L);
     const bool first = s == 0;
     const bool last = s == d->ns - 1;
     const P *tws = tw + d->twoff[s];
-    qc v[KMAX][R];
+    V v[KMAX][R];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
       const int b = t + k * TPT;
 #pragma unroll
       for (int m = 0; m < R; ++m) {
-        qc e = {0, 0};
+        V e = O::zero();
         if (b < NB) {
           if (first) {
             if (valid) e = load(b + m * NB);
           } else {
-            e = O::unpack(lds[lds_pad<PAD>(b + m * NB)]);
+            e = O::in(lds[lds_pad<PAD>(b + m * NB)]);
           }
         }
-        v[k][m] = {O::template rshr<SH>(e.re), O::template rshr<SH>(e.im)};
+        v[k][m] = O::template shr<SH>(e);
       }
     }
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
       const int b = t + k * TPT;
-      const int j = b % L;
+      const int j = b & (L - 1);
       if (!first && b < NB && j > 0) {
 #pragma unroll
-        for (int m = 1; m < R; ++m) v[k][m] = O::cmul(v[k][m], O::unpack(tws[(m - 1) * L + j]));
+        for (int m = 1; m < R; ++m) v[k][m] = O::cmul(v[k][m], tws[(m - 1) * L + j]);
       }
-      O::template dft<R>(v[k], w16);
+      QBfly<O>::template dft<R>(v[k], w16);
     }
     if (!first && !last) __syncthreads();
 #pragma unroll
@@ -183,10 +251,10 @@ struct FixedEngine {
             for (int m = 0; m < R; ++m) store(b + m * NB, v[k][m]);
           }
         } else {
-          const int j = b % L;
-          const int base = (b / L) * L * R + j;
+          const int j = b & (L - 1);
+          const int base = ((b >> lgL) << lgL) * R + j;
 #pragma unroll
-          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = O::pack(v[k][m]);
+          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = O::out(v[k][m]);
         }
       }
     }
@@ -237,18 +305,21 @@ __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   P *out = static_cast<P *>(a.out) + ooff;
   const bool bwd = a.flags & 1;
   const int64_t is = a.istride, os = a.ostride;
-  FixedEngine<BITS, TPT, PTS>::run(
-      a.desc, lds, tab + 16, tab, t, valid,
-      [&](int e) {
-        const P p = in[e * is];
-        return bwd ? qc{(int)p.im, (int)p.re} : qc{(int)p.re, (int)p.im};
-      },
-      [&](int e, qc v) {
-        P p;
-        p.re = (decltype(p.re))(bwd ? v.im : v.re);
-        p.im = (decltype(p.im))(bwd ? v.re : v.im);
-        out[e * os] = p;
-      });
+  using O = typename FixedEngine<BITS, TPT, PTS>::O;
+  using V = typename O::V;
+  // unit strides (the common layout) index without 64-bit multiplies
+  auto go = [&](auto unit) {
+    constexpr bool U = decltype(unit)::value;
+    FixedEngine<BITS, TPT, PTS>::run(
+        a.desc, lds, tab + 16, tab, t, valid,
+        [&](int e) {
+          const V v = O::in(in[U ? (int64_t)e : e * is]);
+          return bwd ? O::swap(v) : v;
+        },
+        [&](int e, V v) { out[U ? (int64_t)e : e * os] = O::out(bwd ? O::swap(v) : v); });
+  };
+  if (is == 1 && os == 1) go(std::true_type{});
+  else go(std::false_type{});
 }
 
 // Real <-> complex staging for fixed-point R2C / C2R through a complex

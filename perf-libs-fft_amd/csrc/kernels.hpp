@@ -137,6 +137,32 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2c_unit(KArgs a) {
       [&](int e, cpx<C> v) { ST::stnt(out + e, BWD ? cswap(v) : v); });
 }
 
+// k_c2c_unit with at least WAVES waves per SIMD: the register budget is cut
+// to 512 / WAVES so that more than one large block fits a CU (for 16384
+// points on the split exchange: two 1024-thread blocks at 64 registers).
+template <typename S, bool BWD, int TPB, class Eng, int WAVES>
+__global__ void __launch_bounds__(Eng::TPT *TPB) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_c2c_unit_occ(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  constexpr int TPT = Eng::TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const E *in = static_cast<const E *>(a.in) + (valid ? tr * a.idist : 0);
+  E *out = static_cast<E *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        cpx<C> v = ST::ldnt(in + e);
+        return BWD ? cswap(v) : v;
+      },
+      [&](int e, cpx<C> v) { ST::stnt(out + e, BWD ? cswap(v) : v); });
+}
+
 // batched C2C, unit element stride, software-pipelined: a block walks the
 // members blockIdx.x, +gridDim.x, ... and issues the stage-0 loads of its
 // next member before running the LDS stages of the current one, so each

@@ -31,6 +31,12 @@ void register_kernels_f32_unit(std::vector<KEntry> &r) {
   PLF_UNIT_SPLIT(S, P, 512, 1, 5, 16, 16, 8, 4);
   PLF_UNIT_SPLIT(S, P, 256, 1, 5, 16, 16, 16);
   PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 16, 16, 16, 8);
+  // two 16384-point blocks per CU: 64 KB of LDS and 64 registers each
+  // (a few registers spill); measured 4240-4420 GB/s against 4000 for the
+  // one-block-per-CU kernel. The same cap at 4096 and 8192 points was
+  // measured 6-9% slower than their uncapped kernels.
+  PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 16, 16, 16, 4);
+  PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 4, 16, 16, 16);
 }
 
 }  // namespace plf

@@ -114,6 +114,22 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
           &::plf::KERNEL<S, false, TPB, ::plf::DEng<typename ::plf::StorageT<S>::C, TPT, PTS, TPB>>, \
           TPT * TPB>)
 
+// Split-exchange unit-stride C2C with at least WAVES waves per SIMD.
+#define PLF_UNIT_SPLIT_OCC(S, PREC, TPT, TPB, PAD, WAVES, ...)                                \
+  do {                                                                                    \
+    using EngT = ::plf::SEngS<typename ::plf::StorageT<S>::C, TPT, PAD, __VA_ARGS__>;      \
+    ::plf::add_entry(                                                                     \
+        r, ::plf::KK_C2C_UNIT, PREC, ::plf::Chain<__VA_ARGS__>::N, {__VA_ARGS__}, TPT, TPB, 0, \
+        ::plf::Chain<__VA_ARGS__>::count > 1,                                             \
+        &::plf::launch_k<&::plf::k_c2c_unit_occ<S, false, TPB, EngT, WAVES>, TPT * TPB>,  \
+        &::plf::launch_k<&::plf::k_c2c_unit_occ<S, true, TPB, EngT, WAVES>, TPT * TPB>);  \
+    for (int i = 1; i <= 2; ++i) {                                                        \
+      r[r.size() - i].pad = PAD;                                                          \
+      r[r.size() - i].lds_scalars = 1;                                                    \
+      r[r.size() - i].waves = WAVES;                                                      \
+    }                                                                                     \
+  } while (0)
+
 // Single-direction kind on the split (real, then imaginary) LDS exchange.
 #define PLF_STATIC1_SPLIT(KERNEL, KK, S, PREC, TPT, TPB, PAD, ...)                         \
   do {                                                                                    \

@@ -55,6 +55,7 @@ std::string KEntry::name() const {
   if (kind == KK_C2C_UNIT) o << "_p" << pad << (pts ? "_pair" : "");
   if (lds_scalars == 1) o << "_split";
   if (persist) o << "_pipe" << persist;
+  if (waves) o << "_w" << waves;
   return o.str();
 }
 

@@ -299,12 +299,15 @@ __device__ __forceinline__ uint16_t f2h(float f) {  // round to nearest even
 __device__ __forceinline__ float bf2f(uint16_t b) {
   return __uint_as_float(uint32_t(b) << 16);
 }
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu))
-    return uint16_t((u >> 16) | 0x40u);  // quiet NaN
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
+// float -> bf16, round to nearest even: gfx950's v_cvt_pk_bf16_f32
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+}
+// two floats -> packed bf16 pair (lo in bits 0..15), one instruction
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
 }
 
 // Storage traits: E is one complex element in memory, R the real scalar.
@@ -380,8 +383,7 @@ struct StorageT<bf16_pair> {
     return {bf2f(uint16_t(u & 0xffffu)), bf2f(uint16_t(u >> 16))};
   }
   static __device__ __forceinline__ void stnt(E *p, cpx<float> c) {
-    const uint32_t u = uint32_t(f2bf(c.x)) | (uint32_t(f2bf(c.y)) << 16);
-    __builtin_nontemporal_store(u, reinterpret_cast<uint32_t *>(p));
+    __builtin_nontemporal_store(f2bf2(c.x, c.y), reinterpret_cast<uint32_t *>(p));
   }
   static __device__ __forceinline__ cpx<float> ld(const E &e) {
     return {bf2f(e.x), bf2f(e.y)};

@@ -35,6 +35,11 @@ void register_kernels_f16(std::vector<KEntry> &r) {
   PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
   PLF_UNIT(S, P, 256,   1, 4, false, 16, 16, 16);
   PLF_UNIT(S, P, 1024,  1, 4, false, 16, 16, 16, 4);
+  // split (real, then imaginary) LDS exchange: more blocks per CU
+  PLF_UNIT_SPLIT(S, P, 256, 1, 5, 16, 16, 16);
+  PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 16, 16, 16, 4);
+  // 16384 at 64 registers (8 waves per SIMD): two blocks per CU
+  PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 16, 16, 16, 4);
 }
 
 }  // namespace plf

@@ -19,6 +19,8 @@
 // (docs/plfft.md:136-170).
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.hpp"
 
 namespace plf {
@@ -56,11 +58,13 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even(KArgs a) {
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
   // flags bit1: interior bins doubled (the adjoint of C2R)
   const C h = (a.flags & 2) ? C(1) : C(0.5);
-  auto post = [&](int k, cpx<C> w) {
+  // output bins k and m-k from Z[k], Z[m-k]; so = output stride (1 when
+  // known at compile time)
+  auto post = [&](int k, cpx<C> w, int64_t so) {
     if (k == 0) {
       const cpx<C> z0 = lds[lds_pad<Eng::PAD>(0)];
       out[0] = ST::st({z0.x + z0.y, C(0)});
-      out[(int64_t)m * os] = ST::st({z0.x - z0.y, C(0)});
+      out[(int64_t)m * so] = ST::st({z0.x - z0.y, C(0)});
       return;
     }
     const cpx<C> A = lds[lds_pad<Eng::PAD>(k)];
@@ -68,13 +72,15 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even(KArgs a) {
     const cpx<C> xe = {h * (A.x + B.x), h * (A.y - B.y)};
     const cpx<C> xo = {h * (A.y + B.y), h * (B.x - A.x)};  // -i (A - conj B) / 2
     const cpx<C> wx = cmul(w, xo);
-    out[(int64_t)k * os] = ST::st(cadd(xe, wx));
-    if (2 * k != m) out[(int64_t)(m - k) * os] = ST::st({xe.x - wx.x, wx.y - xe.y});
+    out[(int64_t)k * so] = ST::st(cadd(xe, wx));
+    if (2 * k != m) out[(int64_t)(m - k) * so] = ST::st({xe.x - wx.x, wx.y - xe.y});
   };
   constexpr int NF = Eng::NFIX;
   if constexpr (NF > 0) {
     // compile-time length: the post-twiddles are fetched before the FFT so
-    // that their latency hides behind it, and the epilogue is unrolled
+    // that their latency hides behind it, and the epilogue is unrolled.
+    // Unit strides (paired real loads, unit output stride) take a copy
+    // without stride arithmetic.
     constexpr int KP = (NF / 2 + TPT) / TPT;  // ceil((NF/2 + 1) / TPT)
     cpx<C> wk[KP];
 #pragma unroll
@@ -82,21 +88,30 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even(KArgs a) {
       const int k = t + i * TPT;
       wk[i] = (2 * k <= NF) ? W[k] : cpx<C>{C(0), C(0)};
     }
-    Eng::template run<true>(
-        a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
-        [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, [&](int, cpx<C>) {});
-    if (!valid) return;
+    auto body = [&](auto unit) {
+      constexpr bool U = decltype(unit)::value;
+      Eng::template run<true>(
+          a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+          [&](int e) {
+            if constexpr (U) return ST::ld(reinterpret_cast<const E *>(x)[e]);
+            else return load_pair<ST>(x, e, is, pair_ok);
+          },
+          [&](int, cpx<C>) {});
+      if (!valid) return;
 #pragma unroll
-    for (int i = 0; i < KP; ++i) {
-      const int k = t + i * TPT;
-      if (2 * k <= NF) post(k, wk[i]);
-    }
+      for (int i = 0; i < KP; ++i) {
+        const int k = t + i * TPT;
+        if (2 * k <= NF) post(k, wk[i], U ? int64_t(1) : os);
+      }
+    };
+    if (pair_ok && os == 1) body(std::true_type{});
+    else body(std::false_type{});
   } else {
     Eng::template run<true>(
         a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
         [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, [&](int, cpx<C>) {});
     if (!valid) return;
-    for (int k = t; 2 * k <= m; k += TPT) post(k, W[k]);
+    for (int k = t; 2 * k <= m; k += TPT) post(k, W[k], os);
   }
 }
 
@@ -132,10 +147,11 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
       y[(int64_t)(2 * e + 1) * os] = ST::str(v.y);
     }
   };
-  // Z'[e] for stage 0, w = W_n^e
-  auto pre = [&](int e, cpx<C> w) {
-    cpx<C> A = ST::ld(X[(int64_t)e * is]);
-    cpx<C> B = ST::ld(X[(int64_t)(m - e) * is]);
+  // Z'[e] for stage 0, w = W_n^e; si = input stride (1 when known at
+  // compile time)
+  auto pre = [&](int e, cpx<C> w, int64_t si) {
+    cpx<C> A = ST::ld(X[(int64_t)e * si]);
+    cpx<C> B = ST::ld(X[(int64_t)(m - e) * si]);
     if (e == 0) {
       A.y = C(0);
       B.y = C(0);
@@ -160,21 +176,27 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
     constexpr int NB = NF / R0;
     constexpr int KB = (NB + TPT - 1) / TPT;
     cpx<C> vin[KB * R0];
+    auto build = [&](auto unit) {
+      constexpr bool U = decltype(unit)::value;  // unit input stride
 #pragma unroll
-    for (int k = 0; k < KB; ++k) {
-      const int b = t + k * TPT;
-      const bool act = valid && ((NB % TPT == 0) || (b < NB));
-      const cpx<C> wb = act ? W[b] : cpx<C>{C(1), C(0)};
+      for (int k = 0; k < KB; ++k) {
+        const int b = t + k * TPT;
+        const bool act = valid && ((NB % TPT == 0) || (b < NB));
+        const cpx<C> wb = act ? W[b] : cpx<C>{C(1), C(0)};
 #pragma unroll
-      for (int j = 0; j < R0; ++j)
-        vin[k * R0 + j] =
-            act ? pre(b + j * NB, j == 0 ? wb : cmul(wb, W[j * NB])) : cpx<C>{C(0), C(0)};
-    }
+        for (int j = 0; j < R0; ++j)
+          vin[k * R0 + j] = act ? pre(b + j * NB, j == 0 ? wb : cmul(wb, W[j * NB]),
+                                      U ? int64_t(1) : is)
+                                : cpx<C>{C(0), C(0)};
+      }
+    };
+    if (is == 1) build(std::true_type{});
+    else build(std::false_type{});
     Eng::E::run_regs(vin, lds, static_cast<const cpx<C> *>(a.tw), t, store, valid);
   } else {
     Eng::template run<false>(
         a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
-        [&](int e) { return pre(e, W[e]); }, store);
+        [&](int e) { return pre(e, W[e], is); }, store);
   }
 }
 
@@ -314,11 +336,17 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_dct2_even(KArgs a) {
   const int nn = NF > 0 ? 2 * NF : n;
   if (valid) {
     C *lr = reinterpret_cast<C *>(lds);
-#pragma unroll 8
-    for (int o = t; o < nn; o += TPT) {
-      const C v = ST::ldr(x[(int64_t)o * is]);
+    auto put = [&](int o, C v) {
       const int i = (o & 1) ? n - (o + 1) / 2 : o / 2;
       lr[2 * lds_pad<Eng::PAD>(i >> 1) + (i & 1)] = (DST && (o & 1)) ? -v : v;
+    };
+    if (is == 1) {
+      // unit stride: every load of the thread in flight at once, addressed
+      // from one base
+      gather_then<TPT, 2 * NF, C>(t, nn, [&](int o) { return ST::ldr(x[o]); }, put);
+    } else {
+#pragma unroll 8
+      for (int o = t; o < nn; o += TPT) put(o, ST::ldr(x[(int64_t)o * is]));
     }
   }
   __syncthreads();
@@ -443,9 +471,15 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_dct3_even(KArgs a) {
     // W_n^e = W_n^b W_n^{j NB}, w_e = w_b w_{j NB} and w_{m-e} = w_m conj(w_e)
     C *lu = reinterpret_cast<C *>(lds);
     if (valid) {
-      gather_then<TPT, 2 * NF, C>(
-          t, 2 * NF, [&](int k) { return dct3_in<ST, DST>(x, k, n, is, s0); },
-          [&](int k, C v) { lu[k] = v; });
+      if (is == 1) {  // unit stride: loads addressed from one base
+        gather_then<TPT, 2 * NF, C>(
+            t, 2 * NF, [&](int k) { return ST::ldr(x[DST ? n - 1 - k : k]); },
+            [&](int k, C v) { lu[k] = k == 0 ? s0 * v : v; });
+      } else {
+        gather_then<TPT, 2 * NF, C>(
+            t, 2 * NF, [&](int k) { return dct3_in<ST, DST>(x, k, n, is, s0); },
+            [&](int k, C v) { lu[k] = v; });
+      }
     }
     __syncthreads();
     using Ch = typename Eng::Ch;
@@ -493,13 +527,16 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_dct3_even(KArgs a) {
   // i = o/2 (o even) or n - (o+1)/2 (o odd, negated for DST)
   const int nn = NF > 0 ? 2 * NF : n;
   const C *lr = reinterpret_cast<const C *>(lds);
-  gather_then<TPT, 2 * NF, C>(
-      t, nn,
-      [&](int o) {
-        const int i = (o & 1) ? n - (o + 1) / 2 : o / 2;
-        return lr[2 * lds_pad<Eng::PAD>(i >> 1) + 1 - (i & 1)];
-      },
-      [&](int o, C v) { y[(int64_t)o * os] = ST::str((DST && (o & 1)) ? -v : v); });
+  auto get = [&](int o) {
+    const int i = (o & 1) ? n - (o + 1) / 2 : o / 2;
+    return lr[2 * lds_pad<Eng::PAD>(i >> 1) + 1 - (i & 1)];
+  };
+  if (os == 1)
+    gather_then<TPT, 2 * NF, C>(t, nn, get,
+                                [&](int o, C v) { y[o] = ST::str((DST && (o & 1)) ? -v : v); });
+  else
+    gather_then<TPT, 2 * NF, C>(
+        t, nn, get, [&](int o, C v) { y[(int64_t)o * os] = ST::str((DST && (o & 1)) ? -v : v); });
 }
 
 template <typename S, bool DST, int TPB, class Eng>

@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even(KArgs a) {
   auto store = [&](int e, cpx<C> v) {
     v = cswap(v);
     if (pair_ok) {
-      reinterpret_cast<E *>(y)[e] = ST::st(v);
+      ST::stnt(reinterpret_cast<E *>(y) + e, v);  // streaming: written once, not re-read
     } else {
       y[(int64_t)(2 * e) * os] = ST::str(v.x);
       y[(int64_t)(2 * e + 1) * os] = ST::str(v.y);

@@ -254,7 +254,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (--gpus {args.gpus} requested): the batch "
-              "shards across GPUs with no communication, see DESIGN.md", file=sys.stderr)
+              "would shard contiguously across GPUs, one process each, with no "
+              "communication; only the one-GPU plan is built and measured here", file=sys.stderr)
         return 2
 
     import torch

@@ -18,6 +18,7 @@
 
 #include "../../include/plfft.h"
 #include "../../include/plfft_gpu.h"
+#include "abi_plan.hpp"
 #include "mixed_plan.hpp"
 #include "plan.hpp"
 
@@ -41,29 +42,6 @@ struct plfft_config_t {
   plfft_io_alias_t alias;
   plfft_sme_mode_t sme_mode;
   plfft_rigor_t rigor;
-};
-
-struct plfft_plan_t {
-  plf::Problem prob;
-  int device = 0;
-  std::unique_ptr<plf::Node> node;
-  // adjoint node, built on first use
-  mutable std::mutex adj_mu;
-  mutable std::unique_ptr<plf::Node> adj;
-  mutable bool adj_tried = false;
-  // workspace pool for concurrent synchronous executions
-  mutable std::mutex ws_mu;
-  mutable std::vector<void *> ws_free;
-  mutable void *ws_default = nullptr;
-  size_t ws_bytes = 0;
-  // element sizes (bytes) of the input and output sides
-  size_t in_elem = 0, out_elem = 0;
-  int64_t in_count = 0, out_count = 0;  // logical elements per transform
-
-  ~plfft_plan_t() {
-    for (void *p : ws_free) (void)hipFree(p);
-    if (ws_default) (void)hipFree(ws_default);
-  }
 };
 
 [[noreturn]] static void out_of_memory() {
@@ -310,7 +288,7 @@ extern "C" plfft_status_t plfft_plan_create_from_config(plfft_plan_t **plan, con
   plfft_plan_t *pl = new (mem) plfft_plan_t;
   pl->prob = p;
   pl->device = dev;
-  pl->ws_bytes = node->workspace_bytes();
+  pl->ws.bytes = node->workspace_bytes();
   pl->node = std::move(node);
   side_sizes(p, &pl->in_elem, &pl->out_elem, &pl->in_count, &pl->out_count);
   plf::set_last_error("");
@@ -329,19 +307,7 @@ extern "C" void plfft_plan_destroy(plfft_plan_t *plan) {
 // ---------------------------------------------------------------------------
 namespace {
 
-struct DeviceGuard {
-  int prev = -1;
-  explicit DeviceGuard(int dev) {
-    int cur = 0;
-    if (hipGetDevice(&cur) == hipSuccess && cur != dev) {
-      prev = cur;
-      (void)hipSetDevice(dev);
-    }
-  }
-  ~DeviceGuard() {
-    if (prev >= 0) (void)hipSetDevice(prev);
-  }
-};
+using plf::DeviceGuard;
 
 bool is_device_pointer(const void *p) {
   hipPointerAttribute_t a;
@@ -354,25 +320,25 @@ bool is_device_pointer(const void *p) {
   return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
 }
 
-void *acquire_ws(const plfft_plan_t *pl) {
-  if (pl->ws_bytes == 0) return nullptr;
+void *acquire_ws(plf::WsPool &pool) {
+  if (pool.bytes == 0) return nullptr;
   {
-    std::lock_guard<std::mutex> g(pl->ws_mu);
-    if (!pl->ws_free.empty()) {
-      void *p = pl->ws_free.back();
-      pl->ws_free.pop_back();
+    std::lock_guard<std::mutex> g(pool.mu);
+    if (!pool.free_list.empty()) {
+      void *p = pool.free_list.back();
+      pool.free_list.pop_back();
       return p;
     }
   }
   void *p = nullptr;
-  if (hipMalloc(&p, pl->ws_bytes) != hipSuccess) out_of_memory();
+  if (hipMalloc(&p, pool.bytes) != hipSuccess) out_of_memory();
   return p;
 }
 
-void release_ws(const plfft_plan_t *pl, void *p) {
+void release_ws(plf::WsPool &pool, void *p) {
   if (!p) return;
-  std::lock_guard<std::mutex> g(pl->ws_mu);
-  pl->ws_free.push_back(p);
+  std::lock_guard<std::mutex> g(pool.mu);
+  pool.free_list.push_back(p);
 }
 
 [[noreturn]] void fatal_hip(const char *what, hipError_t e) {
@@ -386,13 +352,14 @@ size_t span_bytes(int64_t count, int64_t stride, int64_t howmany, int64_t dist, 
   return (size_t)span * elem;
 }
 
-void run_sync(const plfft_plan_t *pl, const plf::Node *node, const void *in, void *out,
+void run_sync(const plfft_plan_t *pl, const plf::Node *node, plf::WsPool &pool, const void *in,
+              void *out,
               size_t in_elem, int64_t in_count, int64_t is, int64_t id, size_t out_elem,
               int64_t out_count, int64_t os, int64_t od) {
   DeviceGuard guard(pl->device);
   const int64_t hm = pl->prob.howmany;
   const bool din = is_device_pointer(in), dout = is_device_pointer(out);
-  void *ws = acquire_ws(pl);
+  void *ws = acquire_ws(pool);
   hipStream_t s = nullptr;  // legacy default stream: ordered after other blocking streams
   hipError_t e;
   if (din && dout) {
@@ -440,35 +407,36 @@ void run_sync(const plfft_plan_t *pl, const plf::Node *node, const void *in, voi
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) fatal_hip("execute", e);
-  release_ws(pl, ws);
+  release_ws(pool, ws);
 }
 
 }  // namespace
 
 extern "C" void plfft_plan_execute(const plfft_plan_t *plan, const void *input, void *output) {
   const plf::Problem &p = plan->prob;
-  run_sync(plan, plan->node.get(), input, output, plan->in_elem, plan->in_count, p.istride,
+  run_sync(plan, plan->node.get(), plan->ws, input, output, plan->in_elem, plan->in_count,
+           p.istride,
            p.idist, plan->out_elem, plan->out_count, p.ostride, p.odist);
 }
 
 extern "C" size_t plfft_plan_get_workspace_size(const plfft_plan_t *plan) {
-  return plan ? plan->ws_bytes : 0;
+  return plan ? plan->ws.bytes : 0;
 }
 
-static void *default_ws(const plfft_plan_t *pl) {
-  if (pl->ws_bytes == 0) return nullptr;
-  std::lock_guard<std::mutex> g(pl->ws_mu);
-  if (!pl->ws_default) {
-    if (hipMalloc(&pl->ws_default, pl->ws_bytes) != hipSuccess) out_of_memory();
+static void *default_ws(plf::WsPool &pool) {
+  if (pool.bytes == 0) return nullptr;
+  std::lock_guard<std::mutex> g(pool.mu);
+  if (!pool.dflt) {
+    if (hipMalloc(&pool.dflt, pool.bytes) != hipSuccess) out_of_memory();
   }
-  return pl->ws_default;
+  return pool.dflt;
 }
 
 extern "C" plfft_status_t plfft_plan_execute_async(const plfft_plan_t *plan, const void *input,
                                                    void *output, void *workspace, void *stream) {
   if (!plan || !input || !output) return PLFFT_NULL_ARGUMENT;
   DeviceGuard guard(plan->device);
-  void *ws = workspace ? workspace : default_ws(plan);
+  void *ws = workspace ? workspace : default_ws(plan->ws);
   hipError_t e = plan->node->enqueue(input, output, ws, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) {
     plf::set_last_error(std::string("launch failed: ") + hipGetErrorString(e));
@@ -528,6 +496,7 @@ static const plf::Node *adjoint_node(const plfft_plan_t *pl) {
       DeviceGuard guard(pl->device);
       pl->adj = plf::build_node(q, &why);
       if (!pl->adj) plf::set_last_error(why);
+      else pl->adj_ws.bytes = pl->adj->workspace_bytes();
     } else {
       plf::set_last_error("no adjoint for this transform kind");
     }
@@ -542,7 +511,8 @@ extern "C" void plfft_plan_execute_adjoint(const plfft_plan_t *plan, const void 
     std::abort();
   }
   const plf::Problem &p = plan->prob;
-  run_sync(plan, n, input, output, plan->out_elem, plan->out_count, p.ostride, p.odist,
+  run_sync(plan, n, plan->adj_ws, input, output, plan->out_elem, plan->out_count, p.ostride,
+           p.odist,
            plan->in_elem, plan->in_count, p.istride, p.idist);
 }
 
@@ -552,8 +522,16 @@ extern "C" plfft_status_t plfft_plan_execute_adjoint_async(const plfft_plan_t *p
   if (!plan || !input || !output) return PLFFT_NULL_ARGUMENT;
   const plf::Node *n = adjoint_node(plan);
   if (!n) return PLFFT_PLAN_CREATION_FAILED;
+  // a caller workspace is sized by plfft_plan_get_workspace_size(), the
+  // forward node's need
+  if (workspace && plan->adj_ws.bytes > plan->ws.bytes) {
+    plf::set_last_error("the adjoint needs " + std::to_string(plan->adj_ws.bytes) +
+                        " bytes of workspace, more than the plan's " +
+                        std::to_string(plan->ws.bytes) + "; pass NULL to use the plan's own");
+    return PLFFT_PLAN_CREATION_FAILED;
+  }
   DeviceGuard guard(plan->device);
-  void *ws = workspace ? workspace : default_ws(plan);
+  void *ws = workspace ? workspace : default_ws(plan->adj_ws);
   hipError_t e = n->enqueue(input, output, ws, static_cast<hipStream_t>(stream));
   if (e != hipSuccess) {
     plf::set_last_error(std::string("launch failed: ") + hipGetErrorString(e));

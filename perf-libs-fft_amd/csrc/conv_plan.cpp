@@ -194,6 +194,7 @@ class BigBluesteinNode : public Node {
     q.istride = q.ostride = 1;
     q.idist = q.odist = M_;
     q.may_alias = true;
+    q.no_conv = 1;
     inner_ = build_c2c_node(q, why);
     if (!inner_) return hipErrorInvalidValue;
     std::vector<double> c = chirp_table(n_), B;
@@ -274,6 +275,12 @@ std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
     if (c.e)
       return finish(std::make_unique<FusedConvNode>(true, c, p.n, p.n - 1, p.prec, p.howmany,
                                                     p.istride, id, p.ostride, od));
+  }
+  // the padded length 2^k >= 2n-1 must stay far from int64 overflow (and
+  // from any length with a four-step plan)
+  if (p.n > (int64_t(1) << 40)) {
+    *why = "length " + std::to_string(p.n) + " is beyond the chirp-z range";
+    return nullptr;
   }
   // fused Bluestein: a power of two with a compiled chain, else a 7-smooth length
   int64_t P2 = 1;

@@ -32,6 +32,7 @@ struct KArgs {
   int64_t n1, n2;            // four-step split N = n1 * n2
   int64_t bigdist;           // four-step: workspace distance between big transforms
   int shift;                 // four-step twiddle split: W^e = hi[e >> shift] * lo[e & mask]
+  int shift2;                // second split table (real four-step: W_n)
   int flags;
 };
 

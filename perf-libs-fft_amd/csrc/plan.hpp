@@ -33,6 +33,9 @@ struct Problem {
   // 1: plan the adjoint of the transform described above (input/output
   // layouts already swapped by the caller)
   int adjoint = 0;
+  // 1: single-kernel or four-step plans only (the padded inner transform of
+  // a chirp-z plan), so that chirp-z never nests
+  int no_conv = 0;
 };
 
 // Owning device allocation.
@@ -110,6 +113,13 @@ int planning_rigor();
 
 // W_n^k (forward sign) for k in [0, count), interleaved re/im, FP64.
 std::vector<double> root_table(int64_t n, int64_t count);
+// Split table of W_n^e for e in [0, count): lo = W_n^r for r < 2^shift, then
+// hi = W_n^{q 2^shift} for q <= (count-1) >> shift, so that W_n^e =
+// hi[e >> shift] * lo[e & (2^shift - 1)]. Forward sign, interleaved, FP64.
+std::vector<double> split_root_table(int64_t n, int64_t count, int *shift);
+// Big transforms per column/row pass of a four-step plan of length n (the
+// workspace holds that many; about 1 GiB unless PLFFT_FOURSTEP_CHUNK_MB says).
+int64_t fourstep_chunk(int64_t n, int prec, int64_t batch);
 
 // Twiddle table of a Stockham radix chain, FORWARD sign, FP64:
 // for stage s >= 1 with radix R and L = product of earlier radices,

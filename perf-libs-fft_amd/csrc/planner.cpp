@@ -79,6 +79,8 @@ const std::vector<KEntry> &kernel_registry() {
     register_kernels_f32_real(r);
     register_kernels_f64(r);
     register_kernels_f64_real(r);
+    register_kernels_f32_real4s(r);
+    register_kernels_f64_real4s(r);
     register_kernels_f16_real(r);
     register_kernels_bf16_real(r);
     register_kernels_conv(r);
@@ -259,14 +261,17 @@ class FourStepNode : public Node {
     if (err == hipSuccess) err = upload_table(hi_, hi, prec_);
     return err;
   }
-  size_t workspace_bytes() const override {
-    return (size_t)chunk_ * (size_t)(n1_ * n2_) * 2 * prec_compute_bytes(prec_);
-  }
+  size_t per_bytes() const { return (size_t)(n1_ * n2_) * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return (size_t)chunk_ * per_bytes(); }
   hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    return run(in, out, work, chunk_, s);
+  }
+  // column pass then row pass over `chunk` big transforms at a time
+  hipError_t run(const void *in, void *out, void *work, int64_t chunk, hipStream_t s) const {
     const int64_t n = n1_ * n2_;
     const size_t ie = 2 * (size_t)prec_real_bytes(prec_);
-    for (int64_t b0 = 0; b0 < batch_; b0 += chunk_) {
-      const int64_t cb = std::min(chunk_, batch_ - b0);
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk) {
+      const int64_t cb = std::min(chunk, batch_ - b0);
       KArgs a{};
       a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * ie;
       a.out = work;
@@ -325,7 +330,7 @@ class FourStepNode : public Node {
 // MI355X the intermediate is not served from the Infinity Cache at any chunk
 // size measured (32 MiB .. 1 GiB); fewer, larger launches win (65536-point
 // batch of 1 GiB: 2.22 TB/s at 128 MiB, 2.48 TB/s at 1 GiB).
-static int64_t fourstep_chunk(int64_t n, int prec, int64_t batch) {
+int64_t fourstep_chunk(int64_t n, int prec, int64_t batch) {
   int64_t mb = 1024;
   if (const char *s = std::getenv("PLFFT_FOURSTEP_CHUNK_MB")) mb = std::max<int64_t>(1, atoll(s));
   const int64_t per = n * 2 * prec_compute_bytes(prec);
@@ -385,6 +390,12 @@ std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why) {
     return node;
   }
   // 3. chirp-z / Rader
+  if (p.no_conv) {
+    *why = "no single-kernel or four-step plan for the padded length " + std::to_string(p.n) +
+           " of a chirp-z transform (" + prec_name(p.prec) + " four-step covers up to " +
+           (prec_compute_bytes(p.prec) == 8 ? "2^22" : "2^24") + " points)";
+    return nullptr;
+  }
   return build_bluestein(p, why);
 }
 

@@ -3,6 +3,7 @@
 // Even n runs the half-length packed kernels; odd n (or an even n whose
 // half length has no kernel) runs the full-length real-input kernels.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <sstream>
 
@@ -18,6 +19,20 @@ std::vector<double> root_table(int64_t n, int64_t count) {
     t[2 * k] = re;
     t[2 * k + 1] = -im;
   }
+  return t;
+}
+
+std::vector<double> split_root_table(int64_t n, int64_t count, int *shift) {
+  int lg = 0;
+  while ((int64_t(1) << lg) < count) ++lg;
+  *shift = (lg + 1) / 2;
+  const int64_t nlo = int64_t(1) << *shift;
+  const int64_t nhi = ((count - 1) >> *shift) + 1;
+  std::vector<double> t(2 * (size_t)(nlo + nhi));
+  for (int64_t r = 0; r < nlo; ++r) unit_root(r, n, &t[2 * r], &t[2 * r + 1]);
+  for (int64_t q = 0; q < nhi; ++q)
+    unit_root(q * nlo, n, &t[2 * (nlo + q)], &t[2 * (nlo + q) + 1]);
+  for (size_t i = 1; i < t.size(); i += 2) t[i] = -t[i];  // forward sign
   return t;
 }
 
@@ -152,6 +167,138 @@ class RealViaComplexNode : public Node {
   std::unique_ptr<Node> inner_;
 };
 
+// R2C / C2R of even n = 2M through the four-step passes of
+// kernels_real4s.hpp: M = m1 * m2 (m1 even), pass A over columns, pass B
+// over rows (R2C: pairs of rows k1, m1 - k1 per slot), `chunk` members per
+// pass through a workspace of chunk * M complex values.
+class RealFourStepNode : public Node {
+ public:
+  RealFourStepNode(bool r2c, Choice col, Choice row, int64_t m1, int64_t m2, int prec,
+                   int64_t batch, int64_t is, int64_t id, int64_t os, int64_t od, int64_t chunk,
+                   int extra_flags)
+      : r2c_(r2c), col_(std::move(col)), row_(std::move(row)), m1_(m1), m2_(m2), prec_(prec),
+        batch_(batch), is_(is), id_(id), os_(os), od_(od), chunk_(chunk), extra_(extra_flags) {}
+  hipError_t init() {
+    hipError_t err = ecol_.init(col_, m1_);
+    if (err == hipSuccess) err = erow_.init(row_, m2_);
+    if (err != hipSuccess) return err;
+    const int64_t M = m1_ * m2_;
+    err = upload_table(wm_, split_root_table(M, M, &shift_m_), prec_);
+    if (err == hipSuccess) err = upload_table(wn_, split_root_table(2 * M, M, &shift_n_), prec_);
+    return err;
+  }
+  size_t per_bytes() const { return (size_t)(m1_ * m2_) * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return (size_t)chunk_ * per_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    return run(in, out, work, chunk_, s);
+  }
+  hipError_t run(const void *in, void *out, void *work, int64_t chunk, hipStream_t s) const {
+    if (batch_ <= 0) return hipSuccess;
+    const int64_t M = m1_ * m2_;
+    const size_t rb = (size_t)prec_real_bytes(prec_);
+    const size_t ib = r2c_ ? rb : 2 * rb, ob = r2c_ ? 2 * rb : rb;
+    // the real side as pairs: unit stride, even distance, pair-aligned base
+    const void *rp = r2c_ ? in : out;
+    const int64_t rs = r2c_ ? is_ : os_, rd = r2c_ ? id_ : od_;
+    const int pair = (rs == 1 && (batch_ == 1 || rd % 2 == 0) &&
+                      ((uintptr_t)rp % (2 * rb)) == 0) ? 1 : 0;
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk) {
+      const int64_t cb = std::min(chunk, batch_ - b0);
+      KArgs a{};
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * ib;
+      a.out = work;
+      a.tw = ecol_.tw.p;
+      a.desc = ecol_.d();
+      a.aux0 = wm_.p;
+      a.aux1 = wn_.p;
+      a.count = cb * m2_;
+      a.istride = is_;
+      a.idist = id_;
+      a.n = m1_;
+      a.n1 = m1_;
+      a.n2 = m2_;
+      a.bigdist = M;
+      a.shift = shift_m_;
+      a.shift2 = shift_n_;
+      a.flags = pair | extra_;
+      hipError_t err = col_.e->launch(a, (a.count + col_.e->tpb - 1) / col_.e->tpb,
+                                      col_.e->lds_bytes(m1_), s);
+      if (err != hipSuccess) return err;
+      KArgs b{};
+      b.in = work;
+      b.out = static_cast<char *>(out) + (size_t)(b0 * od_) * ob;
+      b.tw = erow_.tw.p;
+      b.desc = erow_.d();
+      b.aux1 = wn_.p;
+      b.ostride = os_;
+      b.odist = od_;
+      b.n = m2_;
+      b.n1 = m1_;
+      b.n2 = m2_;
+      b.bigdist = M;
+      b.shift2 = shift_n_;
+      b.flags = pair | extra_;
+      int64_t grid;
+      const int tpb = row_.e->tpb;
+      if (r2c_) {
+        b.count = cb;
+        grid = cb * ((m1_ / 2 + tpb / 2 - 1) / (tpb / 2));
+      } else {
+        b.count = cb * m1_;
+        grid = (b.count + tpb - 1) / tpb;
+      }
+      err = row_.e->launch(b, grid, row_.e->lds_bytes(m2_), s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(" << (r2c_ ? "r2c" : "c2r") << (extra_ & 2 ? "_adjoint" : "") << "_fourstep n="
+      << 2 * m1_ * m2_ << " half=" << m1_ << "x" << m2_ << " chunk=" << chunk_
+      << " batch=" << batch_ << " is=" << is_ << " id=" << id_ << " os=" << os_
+      << " od=" << od_ << " " << chain_str(col_) << " " << chain_str(row_) << ")";
+    return o.str();
+  }
+
+ private:
+  bool r2c_;
+  Choice col_, row_;
+  int64_t m1_, m2_;
+  int prec_;
+  int64_t batch_, is_, id_, os_, od_, chunk_;
+  int extra_;
+  EngineData ecol_, erow_;
+  DevMem wm_, wn_;
+  int shift_m_ = 0, shift_n_ = 0;
+};
+
+// The four-step split M = m1 * m2 (m1 even, kernels for both sides) with
+// compile-time chains preferred, then the most balanced; m1 == 0 if none.
+static void real_fourstep_split(bool r2c, int64_t M, int prec, int64_t *m1_out, Choice *cc,
+                                Choice *rc) {
+  *m1_out = 0;
+  const int kc = r2c ? KK_COL_R2C : KK_COL_C2R, kr = r2c ? KK_ROW_R2C : KK_ROW_C2R;
+  for (int64_t m1 = 2; m1 < M && m1 * m1 <= M * 4; m1 += 2) {
+    if (M % m1) continue;
+    const int64_t m2 = M / m1;
+    Choice c = choose(kc, m1, prec, false);
+    if (!c.e) continue;
+    Choice r = choose(kr, m2, prec, false);
+    if (!r.e) continue;
+    const double bal = std::fabs(std::log((double)m1 / (double)m2));
+    const bool stat = c.e->n && r.e->n;
+    const bool best_stat = *m1_out && cc->e->n && rc->e->n;
+    const double best_bal =
+        *m1_out ? std::fabs(std::log((double)*m1_out / (double)(M / *m1_out))) : 1e30;
+    if (!*m1_out || (stat && !best_stat) || (stat == best_stat && bal < best_bal)) {
+      *m1_out = m1;
+      *cc = c;
+      *rc = r;
+    }
+  }
+}
+
 std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why) {
   const bool r2c = p.kind == K_R2C;
   const int64_t id = p.howmany > 1 ? p.idist : 0;
@@ -161,6 +308,24 @@ std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why) {
   if (p.n % 2 == 0) {
     c = choose(r2c ? KK_R2C_E : KK_C2R_E, p.n / 2, p.prec, false);
     half = c.e != nullptr;
+  }
+  if (!c.e && p.n % 2 == 0 && p.n >= 8) {
+    // packed half-length transform on the four-step passes
+    int64_t m1 = 0;
+    Choice cc, rc;
+    real_fourstep_split(r2c, p.n / 2, p.prec, &m1, &cc, &rc);
+    if (m1) {
+      const int64_t M = p.n / 2;
+      auto node = std::make_unique<RealFourStepNode>(
+          r2c, cc, rc, m1, M / m1, p.prec, p.howmany, p.istride, id, p.ostride, od,
+          fourstep_chunk(M, p.prec, p.howmany), p.adjoint ? 2 : 0);
+      hipError_t err = node->init();
+      if (err != hipSuccess) {
+        *why = std::string("device upload failed: ") + hipGetErrorString(err);
+        return nullptr;
+      }
+      return node;
+    }
   }
   if (!c.e) c = choose(r2c ? KK_R2C_F : KK_C2R_F, p.n, p.prec, false);
   if (!c.e) {

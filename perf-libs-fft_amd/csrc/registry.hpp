@@ -35,12 +35,17 @@ enum KKind : int {
   KK_DCT3_E = 11,  // DCT-III (bwd slot: DST-III), even n, engine length n/2
   KK_DCT3_F = 12,  // DCT-III / DST-III, full-length engine
   KK_C2C_UNIT = 13,  // batched C2C, unit element stride, streaming access
+  KK_COL_R2C = 14,   // R2C of even n, half length M = m1*m2: four-step column pass
+  KK_ROW_R2C = 15,   //   row pass with the fused post-twiddle
+  KK_COL_C2R = 16,   // C2R of even n: column pass with the fused pre-twiddle
+  KK_ROW_C2R = 17,   //   row pass storing real pairs
 };
 inline const char *kkind_name(int k) {
-  static const char *names[] = {"c2c",      "col",      "row",       "r2c_half", "c2r_half",
-                                "r2c_full", "c2r_full", "bluestein", "rader",    "dct2_half",
-                                "dct2_full", "dct3_half", "dct3_full", "c2c_unit"};
-  return (k >= 0 && k <= 13) ? names[k] : "?";
+  static const char *names[] = {"c2c",       "col",       "row",       "r2c_half", "c2r_half",
+                                "r2c_full",  "c2r_full",  "bluestein", "rader",    "dct2_half",
+                                "dct2_full", "dct3_half", "dct3_full", "c2c_unit", "col_r2c",
+                                "row_r2c",   "col_c2r",   "row_c2r"};
+  return (k >= 0 && k <= 17) ? names[k] : "?";
 }
 
 using KLaunch = hipError_t (*)(const KArgs &, int64_t grid, size_t lds, hipStream_t);
@@ -79,6 +84,8 @@ void register_kernels_f32_4s(std::vector<KEntry> &r);
 void register_kernels_f32_unit(std::vector<KEntry> &r);
 void register_kernels_f32_real(std::vector<KEntry> &r);
 void register_kernels_f64_real(std::vector<KEntry> &r);
+void register_kernels_f32_real4s(std::vector<KEntry> &r);
+void register_kernels_f64_real4s(std::vector<KEntry> &r);
 void register_kernels_conv(std::vector<KEntry> &r);
 void register_kernels_r2r(std::vector<KEntry> &r);
 void register_kernels_conv_f32(std::vector<KEntry> &r);

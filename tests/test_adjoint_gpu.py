@@ -24,6 +24,7 @@ def _rdot(a, b):
 CASES = [("c2c", 1024, -1, None), ("c2c", 97, 1, None), ("c2c", 65536, -1, None),
          ("r2c", 4096, -1, None), ("r2c", 1001, -1, None), ("r2c", 31, -1, None),
          ("c2r", 4096, 1, None), ("c2r", 999, 1, None),
+         ("r2c", 65536, -1, None), ("c2r", 65536, 1, None),
          ("r2r", 4096, -1, "dct2"), ("r2r", 4096, -1, "dct3"), ("r2r", 4096, -1, "dst2"),
          ("r2r", 4096, -1, "dst3"), ("r2r", 99, -1, "dct2"), ("r2r", 99, -1, "dst3"),
          ("r2r", 64, -1, "dct1"), ("r2r", 99, -1, "dct1"), ("r2r", 64, -1, "dct4"),

@@ -283,3 +283,20 @@ def test_prime_known_pairs(torch_cuda, n):
     y = from_dev(torch, yd)
     assert np.max(np.abs(y[0] - 1)) < 1e-5
     assert np.max(np.abs(y[1] - np.exp(-2j * np.pi * np.arange(n) * k0 / n))) < 1e-5
+
+
+def test_size_limits_fail_cleanly(torch_cuda):
+    """Lengths past the four-step range of a precision (2^22 points for f64,
+    2^24 for f32) fail plan creation with PLFFT_PLAN_CREATION_FAILED instead of
+    nesting chirp-z plans without end; the largest four-step lengths plan."""
+    for prec, n_ok, n_bad in (("f64", 1 << 22, 1 << 23), ("f32", 1 << 24, 1 << 25)):
+        print(f"size limit {prec}: {n_ok} plans, {n_bad} fails", flush=True)
+        assert "fourstep" in P.Plan("c2c", n_ok, -1, prec, rigor=P.PLFFT_ESTIMATE).describe()
+        with pytest.raises(P.PlfftError) as ei:
+            P.Plan("c2c", n_bad, -1, prec, rigor=P.PLFFT_ESTIMATE)
+        assert ei.value.status == P.PLFFT_PLAN_CREATION_FAILED
+        assert "chirp-z" in P.last_error()
+    # the real transforms of those lengths widen or fold onto the same plans
+    with pytest.raises(P.PlfftError) as ei:
+        P.Plan("r2c", (1 << 24) + 2, -1, "f64", rigor=P.PLFFT_ESTIMATE)
+    assert ei.value.status == P.PLFFT_PLAN_CREATION_FAILED

@@ -157,3 +157,89 @@ def test_differentiable_layout_r2c(torch_cuda, orc):
     n, hm, is_, id_ = 4096, 16, 2, 2 * 4096 + 64
     x, y, ref, plan = _r2c(torch, orc, "f32", n, hm, is_, id_, 1, n // 2 + 1)
     assert relerr(y, ref) < TOL["f32"], plan.describe()
+
+
+# Half lengths beyond one LDS tile run the packed four-step passes
+# (kernels_real4s.hpp): 65536 = 2 x 128 x 256, 2^20 = 2 x 512 x 1024 and
+# 120000 = 2 x 240 x 250 (runtime radix chains).
+FOURSTEP = [65536, 1 << 20, 120000]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", FOURSTEP)
+def test_r2c_c2r_fourstep(torch_cuda, orc, prec, n):
+    torch = torch_cuda
+    hm = 3 if n < (1 << 20) else 2
+    m = n // 2 + 1
+    print(f"r2c/c2r fourstep {prec} n={n} hm={hm}", flush=True)
+    x, y, ref, plan = _r2c(torch, orc, prec, n, hm, 1, n, 1, m)
+    assert "fourstep" in plan.describe(), plan.describe()
+    assert relerr(y, ref) < TOL[prec], plan.describe()
+    z, zref, plan2 = _c2r(torch, orc, prec, n, hm, 1, m, 1, n, ref)
+    assert "fourstep" in plan2.describe(), plan2.describe()
+    assert relerr(z, zref) < TOL[prec], plan2.describe()
+    assert relerr(z / n, x.astype(np.float64)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("n", [65536, 1 << 20])
+def test_r2c_c2r_fourstep_in_place(torch_cuda, orc, prec, n):
+    torch = torch_cuda
+    hm = 2
+    m = n // 2 + 1
+    rdist = 2 * m  # padded real rows, FFTW-style in-place layout
+    print(f"r2c/c2r fourstep in place {prec} n={n}", flush=True)
+    x = rng.normal_f64(5, hm * rdist).astype(RNP[prec])
+    d = to_dev(torch, x)
+    r2c = P.Plan("r2c", n, -1, prec, hm, 1, rdist, 1, m, may_alias=True)
+    assert "fourstep" in r2c.describe()
+    r2c.execute(d, d)
+    Y = from_dev(torch, d).view(CNP[prec])
+    ref = orc.r2c_batched(x.astype(np.float64), n, hm, 1, rdist, 1, m, hm * m)
+    assert relerr(Y, ref) < TOL[prec]
+    P.Plan("c2r", n, 1, prec, hm, 1, m, 1, rdist, may_alias=True).execute(d, d)
+    z = from_dev(torch, d)
+    for b in range(hm):
+        seg = slice(b * rdist, b * rdist + n)
+        assert relerr(z[seg] / n, x[seg].astype(np.float64)) < TOL[prec]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_r2c_c2r_fourstep_strided(torch_cuda, orc, prec):
+    """Unpaired real access: input stride 2 with an odd-padded distance,
+    complex side strided too."""
+    torch = torch_cuda
+    n, hm = 65536, 3
+    is_, id_, os_, od_ = 2, 2 * n + 63, 2, 2 * (n // 2 + 1) + 5
+    x, y, ref, plan = _r2c(torch, orc, prec, n, hm, is_, id_, os_, od_)
+    assert "fourstep" in plan.describe(), plan.describe()
+    assert relerr(y, ref) < TOL[prec], plan.describe()
+    z, zref, _ = _c2r(torch, orc, prec, n, hm, os_, od_, is_, id_, ref)
+    assert relerr(z, zref) < TOL[prec]
+    for b in range(hm):
+        idx = b * id_ + is_ * np.arange(n)
+        assert relerr(z[idx] / n, x[idx].astype(np.float64)) < TOL[prec]
+
+
+def test_r2c_c2r_fourstep_known_pairs(torch_cuda):
+    """Delta -> constant and a pure tone -> one bin (test/delta_constant_test.hpp,
+    test/pure_tone_test.hpp) at n = 2^20."""
+    torch = torch_cuda
+    n = 1 << 20
+    m = n // 2 + 1
+    x = np.zeros(n, np.float64)
+    x[0] = 1.0
+    X = torch.zeros(m, dtype=torch.complex128, device="cuda")
+    P.Plan("r2c", n, -1, "f64").execute(to_dev(torch, x), X)
+    Xh = from_dev(torch, X)
+    assert np.max(np.abs(Xh - 1.0)) < 1e-12
+    k0 = 12345
+    # the phase reduced exactly mod n, so the tone itself carries no
+    # frequency error (2*pi*k0*j/n in floating point leaks ~1e-6 into
+    # neighbouring bins at this n)
+    tone = np.cos(2 * np.pi * ((k0 * np.arange(n, dtype=np.int64)) % n) / n)
+    P.Plan("r2c", n, -1, "f64").execute(to_dev(torch, tone), X)
+    Xh = from_dev(torch, X)
+    want = np.zeros(m, np.complex128)
+    want[k0] = n / 2
+    assert np.max(np.abs(Xh - want)) < 1e-9

@@ -38,6 +38,27 @@ plfft_status_t plfft_plan_execute_async(const plfft_plan_t *plan,
                                         const void *input, void *output,
                                         void *workspace, void *stream);
 
+/* Bytes of the fixed device scratch array behind plfft_plan_execute_noalloc:
+ * 64 MiB on every device that runs it, held until the process ends. */
+#define PLFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. Same transform and stream ordering as
+ * plfft_plan_execute_async(plan, input, output, <workspace>, stream), but the
+ * plan's scratch is a fixed array of PLFFT_NOALLOC_SCRATCH_BYTES bytes per
+ * device inside this library; the call allocates nothing and issues kernel
+ * launches only. Plans that stream their batch through the scratch in
+ * chunks (four-step) shrink the chunk to fit it. `input`/`output` must be
+ * device memory. Every call on one device shares that device's array, so
+ * calls that overlap in time on one device (other streams, other threads)
+ * race on it: order them on one stream or serialize them. Returns
+ * PLFFT_NULL_ARGUMENT (null plan or buffers), PLFFT_INVALID_SIZE when the
+ * plan needs more scratch than the array holds (see plfft_last_error()), or
+ * PLFFT_PLAN_CREATION_FAILED when the scratch lookup or a launch fails. */
+plfft_status_t plfft_plan_execute_noalloc(const plfft_plan_t *plan, const void *input,
+                                          void *output, void *stream);
+
 /* Adjoint of the planned linear map with respect to the real inner product
  * <a, b> = sum Re(a_k conj(b_k)): `input` has the layout of the plan's
  * OUTPUT, `output` the layout of the plan's INPUT (strides, distances and

@@ -58,6 +58,17 @@ class Node {
   virtual hipError_t enqueue(const void *in, void *out, void *work,
                              hipStream_t s) const = 0;
   virtual size_t workspace_bytes() const { return 0; }
+  // Enqueue with only `cap` bytes of workspace. Nodes that stream their
+  // batch through the workspace in chunks shrink the chunk to fit; the
+  // others need workspace_bytes() <= cap. hipErrorInvalidValue (nothing
+  // launched) when it cannot fit.
+  virtual hipError_t enqueue_capped(const void *in, void *out, void *work, size_t cap,
+                                    hipStream_t s) const {
+    if (workspace_bytes() > cap) return hipErrorInvalidValue;
+    return enqueue(in, out, work, s);
+  }
+  // Smallest `cap` that enqueue_capped accepts.
+  virtual size_t min_workspace_bytes() const { return workspace_bytes(); }
   virtual std::string describe() const = 0;
 };
 

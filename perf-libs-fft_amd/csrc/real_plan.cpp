@@ -189,8 +189,15 @@ class RealFourStepNode : public Node {
   }
   size_t per_bytes() const { return (size_t)(m1_ * m2_) * 2 * prec_compute_bytes(prec_); }
   size_t workspace_bytes() const override { return (size_t)chunk_ * per_bytes(); }
+  size_t min_workspace_bytes() const override { return per_bytes(); }
   hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
     return run(in, out, work, chunk_, s);
+  }
+  hipError_t enqueue_capped(const void *in, void *out, void *work, size_t cap,
+                            hipStream_t s) const override {
+    const int64_t fit = (int64_t)(cap / per_bytes());
+    if (fit < 1) return hipErrorInvalidValue;
+    return run(in, out, work, std::min(chunk_, fit), s);
   }
   hipError_t run(const void *in, void *out, void *work, int64_t chunk, hipStream_t s) const {
     if (batch_ <= 0) return hipSuccess;

@@ -53,6 +53,9 @@ PLFFT_SME_DISABLED = 0
 PLFFT_SME_ENABLED = 1
 PLFFT_ESTIMATE, PLFFT_MEASURE, PLFFT_PATIENT, PLFFT_EXHAUSTIVE = 0, 1, 2, 3
 
+# include/plfft_gpu.h: bytes of the fixed scratch behind plfft_plan_execute_noalloc
+PLFFT_NOALLOC_SCRATCH_BYTES = 64 << 20
+
 # Every symbol declared in include/plfft.h and include/plfft_gpu.h.
 EXPORTS = [
     "plfft_status_to_string",
@@ -68,6 +71,7 @@ EXPORTS = [
     "plfft_plan_create_from_config", "plfft_plan_destroy", "plfft_plan_execute",
     # plfft_gpu.h
     "plfft_plan_get_workspace_size", "plfft_plan_execute_async",
+    "plfft_plan_execute_noalloc",
     "plfft_plan_execute_adjoint", "plfft_plan_execute_adjoint_async",
     "plfft_plan_describe", "plfft_last_error",
     "plfft_mixed_plan_create", "plfft_mixed_plan_destroy",
@@ -125,6 +129,7 @@ def load() -> ctypes.CDLL:
         "plfft_plan_execute": (None, [vp, vp, vp]),
         "plfft_plan_get_workspace_size": (ctypes.c_size_t, [vp]),
         "plfft_plan_execute_async": (i32, [vp, vp, vp, vp, vp]),
+        "plfft_plan_execute_noalloc": (i32, [vp, vp, vp, vp]),
         "plfft_plan_execute_adjoint": (None, [vp, vp, vp]),
         "plfft_plan_execute_adjoint_async": (i32, [vp, vp, vp, vp, vp]),
         "plfft_plan_describe": (ctypes.c_size_t, [vp, ctypes.c_char_p, ctypes.c_size_t]),
@@ -253,6 +258,13 @@ class Plan:
                                              _stream_handle(stream))
         if st != PLFFT_OK:
             raise PlfftError(st, "execute_async")
+
+    def execute_noalloc(self, x, y, stream=None):
+        """plfft_plan_execute_noalloc: scratch from the library's fixed device
+        array (PLFFT_NOALLOC_SCRATCH_BYTES), no allocation."""
+        st = load().plfft_plan_execute_noalloc(self.h, _ptr(x), _ptr(y), _stream_handle(stream))
+        if st != PLFFT_OK:
+            raise PlfftError(st, "execute_noalloc")
 
     def adjoint(self, y, x):
         load().plfft_plan_execute_adjoint(self.h, _ptr(y), _ptr(x))

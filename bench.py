@@ -79,7 +79,35 @@ def cpu_baseline(n: int, seconds: float = 10.0):
     gflops = flops_c2c(n, sample) * iters / dt / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
             "sample": f"C2C f32 fwd N={n}, {sample} transforms x {iters} passes "
-                      f"({dt:.1f} s), oracle FP32 port of the reference CT plan, OpenMP"}
+                      f"({dt:.1f} s); CPU restatement, not PLFFT: the oracle's FP32 port "
+                      "of the reference CT plan (the AArch64 reference cannot run on this "
+                      "x86 host), OpenMP over the batch; its direct codelets and radix-r "
+                      "stages are plain O(r^2) loops, so it understates a tuned CPU FFT "
+                      "(see cpu_scipy)"}
+
+
+def cpu_scipy(n: int, seconds: float = 5.0):
+    """scipy.fft (pocketfft) on the same sample, all worker threads: a tuned
+    CPU FFT beside the port (neither is PLFFT)."""
+    import numpy as np
+    import scipy.fft
+
+    from plfft_amd import rng
+
+    threads = min(16, os.cpu_count() or 1)
+    sample = 2048
+    x = rng.complex_uniform(0, n * sample).reshape(sample, n)
+    scipy.fft.fft(x, axis=1, workers=threads)
+    iters, t0 = 0, time.perf_counter()
+    while True:
+        scipy.fft.fft(x, axis=1, workers=threads)
+        iters += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
+    return {"value": round(flops_c2c(n, sample) * iters / dt / 1e9, 3), "unit": "GFLOP/s",
+            "cores": threads, "kind": "scipy.fft (pocketfft), not PLFFT",
+            "sample": f"C2C complex64 fwd N={n}, {sample} transforms x {iters} passes ({dt:.1f} s)"}
 
 
 def time_stream(torch, fn, steps: int, warmup: int):
@@ -236,6 +264,104 @@ def bench_mixed(torch, P, steps: int, warmup: int):
     return res
 
 
+def case_specs(total: int):
+    """The measured cases besides the headline. Each holds 2^27 complex
+    elements of f32 (1 GiB; f64 and the 16-bit / Q15 types: the same bytes)
+    unless its configuration fixes the batch: the reference's CPU-runnable
+    case (1024 x 4096), the prime and mixed-radix lengths (B = floor(2^26/N)),
+    and the differentiable-transform layout (N=4096, B=8192, input stride 2,
+    distance 2N+64)."""
+    def c(kind, n, prec="f32", d=-1, inplace=False, **kw):
+        sp = {"kind": kind, "n": n, "dtype": prec, "dir": d, "inplace": inplace}
+        sp.update(kw)
+        return sp
+
+    specs = [
+        c("c2c", 256), c("c2c", 1024, batch=4096), c("c2c", 4096, d=1),
+        c("c2c", 4096, inplace=True), c("c2c", 16384), c("c2c", 65536),
+        c("c2c", 65536, d=1, inplace=True), c("c2c", 1 << 20),
+        c("c2c", 1 << 20, d=1, inplace=True),
+        c("c2c", 4096, "f64"), c("c2c", 65536, "f64"), c("c2c", 1 << 20, "f64"),
+        c("c2c", 65536, "f64", d=1, inplace=True),
+        c("r2c", 256), c("r2c", 4096, adjoint=True), c("c2r", 4096, d=1, adjoint=True),
+        c("r2c", 65536, adjoint=True), c("c2r", 65536, d=1),
+        c("r2c", 1 << 20), c("c2r", 1 << 20, d=1), c("r2c", 65536, "f64"),
+        c("r2c", 65536, inplace=True),
+        c("c2c", 4096, "q15"), c("c2c", 4096, "f16"),
+        c("r2r", 4096, r2r="dct2", adjoint=True),
+    ]
+    # prime and mixed-radix lengths, B = floor(2^26 / N)
+    for prec in ("f32", "f64"):
+        for n in (17, 97, 1009, 65537, 15015):
+            specs.append(c("c2c", n, prec, batch=(1 << 26) // n))
+    # the differentiable-transform layout: input stride 2, distance 2N+64
+    for kind, r2r in (("r2c", 0), ("r2r", "dct2"), ("r2r", "dst2")):
+        specs.append(c(kind, 4096, batch=8192, istride=2, idist=2 * 4096 + 64, r2r=r2r,
+                       adjoint=True, layout="cfg5"))
+    return specs
+
+
+def run_case(torch, P, sp, x, y, xr, g, args):
+    """Time one case on the resident buffers; adjoint/forward time ratio when
+    the spec asks for it."""
+    kind, cn, prec, cdir, inplace = sp["kind"], sp["n"], sp["dtype"], sp["dir"], sp["inplace"]
+    total = x.numel()
+    esz = {"f64": 16, "q15": 4, "f16": 4, "bf16": 4}.get(prec, 8)  # bytes per complex
+    cb = sp.get("batch") or total * 8 // esz // cn
+    is_ = sp.get("istride", 1)
+    in_count = cn // 2 + 1 if kind == "c2r" else cn
+    out_count = cn // 2 + 1 if kind == "r2c" else cn
+    id_ = sp.get("idist") or in_count * is_
+    rec = {"kind": kind, "n": cn, "dtype": prec, "batch": cb,
+           "dir": "fwd" if cdir < 0 else "bwd",
+           "placement": "in-place" if inplace else "out-of-place"}
+    if sp.get("r2r"):
+        rec["r2r"] = sp["r2r"]
+    if sp.get("layout"):
+        rec["layout"] = f"istride={is_} idist={id_}"
+    if inplace and kind in ("r2c", "c2r"):
+        # FFTW-style padded real rows so that both sides fit one buffer
+        id_ = 2 * (cn // 2 + 1) if kind == "r2c" else cn // 2 + 1
+        od_ = cn // 2 + 1 if kind == "r2c" else 2 * (cn // 2 + 1)
+    else:
+        od_ = out_count
+    progress(f"case {kind} n={cn} {prec} batch={cb} {rec['dir']} {rec['placement']}")
+    try:
+        cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, istride=is_, idist=id_, odist=od_,
+                       may_alias=inplace, r2r=sp.get("r2r") or 0)
+    except P.PlfftError as e:
+        rec["status"] = f"unsupported: {e}"
+        return rec
+    dst = x if inplace else y
+    steps = max(10, args.steps // 2)
+    cms = time_plan(torch, cplan, x, dst, steps, args.warmup)
+    half = esz // 2  # bytes per real
+    if kind == "c2c":
+        cbytes = 2 * cn * esz * cb
+        cfl = flops_c2c(cn, cb)
+    elif kind == "r2r":  # n reals in, n reals out
+        cbytes = 2 * cn * half * cb
+        cfl = flops_c2c(cn, cb) / 2
+    else:
+        cbytes = (cn * half + (cn // 2 + 1) * esz) * cb
+        cfl = flops_c2c(cn, cb) / 2
+    gbs = cbytes / (cms * 1e-3) / 1e9
+    rec.update({"ms": round(cms, 4), "GFLOP/s": round(cfl / (cms * 1e-3) / 1e9, 1),
+                "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                "plan": cplan.describe()})
+    if sp.get("adjoint") and not inplace:
+        # adjoint / forward time: the adjoint reads the forward output's
+        # layout and writes the input's
+        ams = time_plan(torch, cplan, y, x, steps, args.warmup, adjoint=True)
+        rec["adjoint_over_forward"] = round(ams / cms, 3)
+        rec["adjoint_ms"] = round(ams, 4)
+    del cplan
+    if inplace or sp.get("adjoint"):  # restore the synthetic input
+        xr.uniform_(-1.0, 1.0, generator=g)
+        torch.cuda.synchronize()
+    return rec
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,65 +446,8 @@ def main() -> int:
 
     cases = []
     if not args.headline_only:
-        specs = [
-            ("c2c", 256, "f32", -1, False), ("c2c", 1024, "f32", -1, False),
-            ("c2c", 4096, "f32", 1, False), ("c2c", 4096, "f32", -1, True),
-            ("c2c", 16384, "f32", -1, False), ("c2c", 65536, "f32", -1, False),
-            ("c2c", 1 << 20, "f32", -1, False), ("c2c", 4096, "f64", -1, False),
-            ("r2c", 4096, "f32", -1, False), ("c2r", 4096, "f32", 1, False),
-            ("c2c", 4096, "q15", -1, False), ("c2c", 4096, "f16", -1, False),
-            ("r2r", 4096, "f32", -1, False),
-        ]
-        for kind, cn, prec, cdir, inplace in specs:
-            if prec == "f64":
-                cb = total // 2 // cn
-            elif prec in ("q15", "f16"):
-                cb = total * 2 // cn  # same bytes as the f32 cases
-            elif kind in ("r2c", "c2r"):
-                cb = total // cn
-            else:
-                cb = total // cn
-            if cn == 1024 and kind == "c2c" and prec == "f32":
-                cb = 4096  # the reference's CPU-runnable configuration
-            try:
-                cplan = P.Plan(kind, cn, cdir, prec, howmany=cb, may_alias=inplace,
-                               r2r="dct2")
-            except P.PlfftError as e:
-                cases.append({"kind": kind, "n": cn, "dtype": prec, "batch": cb,
-                              "status": f"unsupported: {e}"})
-                continue
-            dst = x if inplace else y
-            cms = time_plan(torch, cplan, x, dst, max(10, args.steps // 2), args.warmup)
-            esz = {"f64": 16, "q15": 4, "f16": 4}.get(prec, 8)
-            if kind == "c2c":
-                cbytes = 2 * cn * esz * cb
-                cfl = flops_c2c(cn, cb)
-            elif kind == "r2r":  # DCT-II: n reals in, n reals out
-                cbytes = 2 * cn * (esz // 2) * cb
-                cfl = flops_c2c(cn, cb) / 2
-            else:
-                cbytes = (cn * esz // 2 + (cn // 2 + 1) * esz) * cb
-                cfl = flops_c2c(cn, cb) / 2
-            gbs = cbytes / (cms * 1e-3) / 1e9
-            adj = None
-            if not inplace and prec == "f32" and cn == 4096:
-                # adjoint / forward time (the adjoint kernels reuse the forward ones)
-                ams = time_plan(torch, cplan, y, x, max(10, args.steps // 2), args.warmup,
-                                adjoint=True)
-                adj = round(ams / cms, 3)
-                xr.uniform_(-1.0, 1.0, generator=g)  # the adjoint wrote into x
-                torch.cuda.synchronize()
-            cases.append({"kind": kind, "n": cn, "dtype": prec, "batch": cb,
-                          "adjoint_over_forward": adj,
-                          "dir": "fwd" if cdir < 0 else "bwd",
-                          "placement": "in-place" if inplace else "out-of-place",
-                          "ms": round(cms, 4), "GFLOP/s": round(cfl / (cms * 1e-3) / 1e9, 1),
-                          "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                          "plan": cplan.describe()})
-            del cplan
-            if inplace:  # restore the synthetic input
-                xr.uniform_(-1.0, 1.0, generator=g)
-                torch.cuda.synchronize()
+        for spec in case_specs(total):
+            cases.append(run_case(torch, P, spec, x, y, xr, g, args))
 
     out = {
         "metric": "batched C2C f32 GFLOP/s (5 N log2 N) and %HBM peak",
@@ -411,6 +480,10 @@ def main() -> int:
             out["cpu_baseline"] = cpu_baseline(n)
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"value": None, "error": str(e)}
+        try:
+            out["cpu_scipy"] = cpu_scipy(n)
+        except Exception as e:
+            out["cpu_scipy"] = {"value": None, "error": str(e)}
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -63,7 +63,7 @@ struct StaticEngine {
         } else if constexpr (FIRST && !LDS_IN) {
           v[k][m] = (act && valid) ? load(b + m * NB) : cpx<C>{C(0), C(0)};
         } else {
-          v[k][m] = act ? lds[lds_pad<PAD>(b + m * NB)] : cpx<C>{C(0), C(0)};
+          v[k][m] = act ? lds[lds_pad_step<PAD, NB>(b, m)] : cpx<C>{C(0), C(0)};
         }
       }
     }
@@ -94,14 +94,14 @@ struct StaticEngine {
       } else if constexpr (LAST) {
         if (act) {
 #pragma unroll
-          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(b + m * NB)] = v[k][m];
+          for (int m = 0; m < R; ++m) lds[lds_pad_step<PAD, NB>(b, m)] = v[k][m];
         }
       } else {
         if (act) {
           const int j = b % L;
           const int base = (b / L) * L * R + j;
 #pragma unroll
-          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = v[k][m];
+          for (int m = 0; m < R; ++m) lds[lds_pad_step<PAD, L>(base, m)] = v[k][m];
         }
       }
     }
@@ -137,8 +137,15 @@ struct StaticEngine {
 // one real array of N (padded) scalars. Half the LDS per transform lets two
 // 16384-point blocks share a CU, so one block's global traffic overlaps the
 // other's exchanges. Global input and output only (no TO_LDS / LDS_IN).
+//
+// MIRROR bit0 (first stage) / bit1 (last stage): the stage's butterflies are
+// handed out in mirrored pairs, slot 2q of thread t holding butterfly
+// lo = t + q*TPT and slot 2q+1 butterfly NB - lo (NB/2 when lo = 0), so that
+// elements e and N - e (mod N) of that stage sit in one thread: the real
+// transforms' Hermitian pre- and post-twiddles then need no exchange. Needs
+// NB = KB * TPT with KB even.
 // ---------------------------------------------------------------------------
-template <typename C, int TPT, int PAD, typename Ch>
+template <typename C, int TPT, int PAD, typename Ch, int MIRROR = 0>
 struct StaticEngineSplit {
   static constexpr int N = Ch::N;
   static constexpr int NS = Ch::count;
@@ -152,6 +159,22 @@ struct StaticEngineSplit {
     static __device__ __forceinline__ bool act(int b) { return (NB % TPT == 0) || (b < NB); }
   };
 
+  // butterfly in slot k of thread t at stage SI
+  template <int SI>
+  static __device__ __forceinline__ int bfly(int t, int k) {
+    using T = St<SI>;
+    constexpr bool MIR = ((MIRROR & 1) && SI == 0) || ((MIRROR & 2) && SI == NS - 1);
+    if constexpr (MIR) {
+      static_assert(T::NB == T::KB * TPT && T::KB % 2 == 0,
+                    "mirrored stage: NB = KB * TPT with KB even");
+      const int lo = t + (k >> 1) * TPT;
+      if (!(k & 1)) return lo;
+      return lo == 0 ? T::NB / 2 : T::NB - lo;
+    } else {
+      return t + k * TPT;
+    }
+  }
+
   // twiddles (all but the first stage) and butterflies of stage SI
   template <int SI>
   static __device__ __forceinline__ void compute(cpx<C> (&v)[St<SI>::KB][St<SI>::R],
@@ -159,7 +182,7 @@ struct StaticEngineSplit {
     using T = St<SI>;
 #pragma unroll
     for (int k = 0; k < T::KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bfly<SI>(t, k);
       if constexpr (SI > 0) {
         const int j = b % T::L;
         if (T::act(b)) {
@@ -179,12 +202,12 @@ struct StaticEngineSplit {
     using T = St<SI>;
 #pragma unroll
     for (int k = 0; k < T::KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bfly<SI>(t, k);
       if (T::act(b)) {
         const int base = (b / T::L) * T::L * T::R + b % T::L;
 #pragma unroll
         for (int m = 0; m < T::R; ++m)
-          lds[lds_pad<PAD>(base + m * T::L)] = COMP ? v[k][m].y : v[k][m].x;
+          lds[lds_pad_step<PAD, T::L>(base, m)] = COMP ? v[k][m].y : v[k][m].x;
       }
     }
   }
@@ -196,26 +219,33 @@ struct StaticEngineSplit {
     using T = St<SI>;
 #pragma unroll
     for (int k = 0; k < T::KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bfly<SI>(t, k);
 #pragma unroll
       for (int m = 0; m < T::R; ++m) {
-        const C x = T::act(b) ? lds[lds_pad<PAD>(b + m * T::NB)] : C(0);
+        const C x = T::act(b) ? lds[lds_pad_step<PAD, T::NB>(b, m)] : C(0);
         if (COMP) v[k][m].y = x;
         else v[k][m].x = x;
       }
     }
   }
 
-  template <int SI, class Store>
+  // KEEP: the last stage's outputs go to keep[k * R + m] (registers)
+  // instead of through `store`
+  template <int SI, class Store, bool KEEP = false>
   static __device__ __forceinline__ void from(cpx<C> (&v)[St<SI>::KB][St<SI>::R], C *lds,
                                               const cpx<C> *__restrict__ tw, int t, bool valid,
-                                              Store &store) {
+                                              Store &store, cpx<C> *keep = nullptr) {
     using T = St<SI>;
     compute<SI>(v, tw, t);
-    if constexpr (SI == NS - 1) {
+    if constexpr (SI == NS - 1 && KEEP) {
+#pragma unroll
+      for (int k = 0; k < T::KB; ++k)
+#pragma unroll
+        for (int m = 0; m < T::R; ++m) keep[k * T::R + m] = v[k][m];
+    } else if constexpr (SI == NS - 1) {
 #pragma unroll
       for (int k = 0; k < T::KB; ++k) {
-        const int b = t + k * TPT;
+        const int b = bfly<SI>(t, k);
         if (T::act(b) && valid) {
 #pragma unroll
           for (int m = 0; m < T::R; ++m) store(b + m * T::NB, v[k][m]);
@@ -231,7 +261,7 @@ struct StaticEngineSplit {
       __syncthreads();
       get<SI + 1, 1>(lds, w, t);
       if constexpr (SI + 1 < NS - 1) __syncthreads();  // the next exchange rewrites LDS
-      from<SI + 1>(w, lds, tw, t, valid, store);
+      from<SI + 1, Store, KEEP>(w, lds, tw, t, valid, store, keep);
     }
   }
 
@@ -243,7 +273,7 @@ struct StaticEngineSplit {
     cpx<C> v[T::KB][T::R];
 #pragma unroll
     for (int k = 0; k < T::KB; ++k) {
-      const int b = t + k * TPT;
+      const int b = bfly<0>(t, k);
 #pragma unroll
       for (int m = 0; m < T::R; ++m)
         v[k][m] = (T::act(b) && valid) ? load(b + m * T::NB) : cpx<C>{C(0), C(0)};
@@ -251,8 +281,28 @@ struct StaticEngineSplit {
     from<0>(v, reinterpret_cast<C *>(lds), tw, t, valid, store);
   }
 
+  // Every stage, with the last stage's outputs left in registers:
+  // keep[k * RL + m] = element b + m * (N / RL) of butterfly b = bfly(t, k)
+  // of the last stage (RL its radix; natural order). LDS may still be being
+  // read by other threads on return: barrier before reusing it.
+  template <class Load>
+  static __device__ __forceinline__ void run_keep(cpx<C> *lds, const cpx<C> *tw, int t,
+                                                  bool valid, Load &&load, cpx<C> *keep) {
+    using T = St<0>;
+    cpx<C> v[T::KB][T::R];
+#pragma unroll
+    for (int k = 0; k < T::KB; ++k) {
+      const int b = bfly<0>(t, k);
+#pragma unroll
+      for (int m = 0; m < T::R; ++m)
+        v[k][m] = (T::act(b) && valid) ? load(b + m * T::NB) : cpx<C>{C(0), C(0)};
+    }
+    auto none = [](int, cpx<C>) {};
+    from<0, decltype(none), true>(v, reinterpret_cast<C *>(lds), tw, t, valid, none, keep);
+  }
+
   // Stage 0 takes its inputs from registers, laid out as for
-  // StaticEngine::run_regs.
+  // StaticEngine::run_regs, with butterfly bfly(t, k) in slot k.
   template <bool TO_LDS = false, class Store>
   static __device__ __forceinline__ void run_regs(const cpx<C> *vin, cpx<C> *lds,
                                                   const cpx<C> *tw, int t, Store &&store,

@@ -436,5 +436,16 @@ template <int PADSHIFT>
 constexpr int lds_padded(int n) {
   return PADSHIFT <= 0 ? n : n + (n >> PADSHIFT) + 1;
 }
+// lds_pad(base + m * STRIDE) for base >= 0. When STRIDE is a multiple of the
+// padding period this is lds_pad(base) plus a constant step per m, which
+// lets the compiler address a whole butterfly from one register with
+// immediate offsets instead of one address register per element.
+template <int PADSHIFT, int STRIDE>
+__device__ __forceinline__ int lds_pad_step(int base, int m) {
+  if constexpr (PADSHIFT > 0 && STRIDE % (1 << PADSHIFT) == 0)
+    return lds_pad<PADSHIFT>(base) + m * (STRIDE + (STRIDE >> PADSHIFT));
+  else
+    return lds_pad<PADSHIFT>(base + m * STRIDE);
+}
 
 }  // namespace plf

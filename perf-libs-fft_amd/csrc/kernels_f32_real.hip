@@ -21,6 +21,11 @@ void register_kernels_f32_real(std::vector<KEntry> &r) {
     // per member, so more blocks share a CU; measured 6% faster than the
     // plain exchange (0.283 vs 0.300 ms for 32768 transforms)
     PLF_STATIC1_SPLIT(k_c2r_even, KK_C2R_E, S, P, 128, 2, 5, 16, 16, 8);
+    // n = 65536: the half-length 32768-point transform held by 1024 threads
+    // (32 points each) on the split exchange (132 KB of LDS), one HBM pass
+    // instead of the two of the four-step passes
+    PLF_STATIC1_SPLIT(k_r2c_even_split, KK_R2C_E, S, P, 1024, 1, 5, 16, 16, 16, 8);
+    PLF_STATIC1_SPLIT(k_c2r_even_split, KK_C2R_E, S, P, 1024, 1, 5, 16, 16, 16, 8);
     PLF_STATIC1(k_c2r_even, KK_C2R_E, S, P, 4, 64, 8, 4);
     PLF_STATIC1(k_c2r_even, KK_C2R_E, S, P, 8, 32, 8, 8);
     PLF_STATIC1(k_c2r_even, KK_C2R_E, S, P, 8, 32, 16, 8);

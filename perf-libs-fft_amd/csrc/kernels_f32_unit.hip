@@ -13,11 +13,19 @@ void register_kernels_f32_unit(std::vector<KEntry> &r) {
   PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
   PLF_UNIT(S, P,  32,   8, 4, true,  16, 16, 4);
   PLF_UNIT(S, P, 128,   2, 4, true,  8, 8, 8, 4);
-  PLF_UNIT(S, P, 256,   1, 4, false, 16, 16, 16);
+  // 4096: the first entry is the ESTIMATE choice and the incumbent of a timed
+  // race; at 2^27 points the p5/p4 plain-exchange kernels measured 0.70 of
+  // the HBM peak, the split exchange 0.68, the pipelined ones 0.61-0.65 and
+  // the 8-point-butterfly and paired ones 0.51-0.54
   PLF_UNIT(S, P, 256,   1, 5, false, 16, 16, 16);
+  PLF_UNIT(S, P, 256,   1, 4, false, 16, 16, 16);
   PLF_UNIT(S, P, 128,   2, 4, true,  16, 16, 16);
   PLF_UNIT(S, P, 256,   1, 4, true,  8, 8, 8, 8);
   PLF_UNIT(S, P, 256,   2, 4, true,  8, 8, 8, 8);
+  // 16384: the split exchange at 64 registers (two blocks per CU) first:
+  // 0.63-0.64 of the HBM peak against 0.62 (one block per CU) and 0.50 for
+  // the plain exchange
+  PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 4, 16, 16, 16);
   PLF_UNIT(S, P, 1024,  1, 4, false, 16, 16, 16, 4);
   PLF_UNIT(S, P, 512,   1, 4, true,  16, 16, 16, 4);
   // software-pipelined persistent blocks (next member's loads in flight);
@@ -36,7 +44,6 @@ void register_kernels_f32_unit(std::vector<KEntry> &r) {
   // one-block-per-CU kernel. The same cap at 4096 and 8192 points was
   // measured 6-9% slower than their uncapped kernels.
   PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 16, 16, 16, 4);
-  PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 4, 16, 16, 16);
 }
 
 }  // namespace plf

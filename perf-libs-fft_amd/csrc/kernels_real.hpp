@@ -116,6 +116,192 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even(KArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// R2C and C2R, even n = 2m, on the split (real, then imaginary) exchange
+// engine with mirrored butterflies (StaticEngineSplit, MIRROR): the whole
+// length-m complex DFT stays in registers and in half the LDS of the plain
+// engine, so m = 32768 runs in one pass with 1024 threads (32 points each).
+// The R2C's last stage and the C2R's first stage hand each thread the
+// butterflies lo = t + q*TPT and NB - lo (NB/2 for lo = 0), so the
+// Hermitian partners Z[e] and Z[m - e] of the post- and pre-twiddles are in
+// the same thread:
+//   e = lo + j*NB  <->  m - e = (NB - lo) + (R-1-j)*NB       (lo > 0)
+//   e = j*NB       <->  m - e = ((R - j) mod R)*NB            (lo = 0)
+//   e = NB/2 + j*NB <-> m - e = NB/2 + (R-1-j)*NB
+// R2C computes X[e] and X[m-e] = conj(Xe - W Xo) from one pair; C2R loads
+// each X[k] once and forms Z'[e] and Z'[m-e] from it. flags and aux0 as for
+// k_r2c_even / k_c2r_even.
+// ---------------------------------------------------------------------------
+template <typename S, bool UNUSED, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even_split(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using Rs = typename ST::Rs;
+  using C = typename Eng::C;
+  using Ch = typename Eng::Ch;
+  constexpr int TPT = Eng::TPT;
+  using EM = StaticEngineSplit<C, TPT, Eng::PAD, Ch, 2>;  // last stage mirrored
+  constexpr int M = Ch::N;
+  constexpr int RL = Ch::radix(Ch::count - 1);
+  constexpr int NB = M / RL;
+  constexpr int KB = NB / TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const Rs *x = static_cast<const Rs *>(a.in) + (valid ? tr * a.idist : 0);
+  E *out = static_cast<E *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(M);
+  const int64_t is = a.istride, os = a.ostride;
+  const bool pair_ok = a.flags & 1;
+  const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
+  const C h = (a.flags & 2) ? C(1) : C(0.5);
+  // post-twiddles of the low butterflies: W_n^{lo + j NB} = W_n^lo W_n^{j NB}
+  cpx<C> wl[KB / 2];
+#pragma unroll
+  for (int q = 0; q < KB / 2; ++q) wl[q] = W[t + q * TPT];
+  cpx<C> z[KB * RL];
+  EM::run_keep(lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+               [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, z);
+  if (!valid) return;
+  auto body = [&](auto unit) {
+    constexpr bool U = decltype(unit)::value;  // unit output stride
+    const int64_t so = U ? int64_t(1) : os;
+    // X[e] = Xe + w Xo and X[m - e] = conj(Xe - w Xo) from A = Z[e], B = Z[m-e]
+    auto pair = [&](int e, cpx<C> A, cpx<C> B, cpx<C> w, bool both) {
+      const cpx<C> xe = {h * (A.x + B.x), h * (A.y - B.y)};
+      const cpx<C> xo = {h * (A.y + B.y), h * (B.x - A.x)};  // -i (A - conj B) h
+      const cpx<C> wx = cmul(w, xo);
+      out[(int64_t)e * so] = ST::st(cadd(xe, wx));
+      if (both) out[(int64_t)(M - e) * so] = ST::st({xe.x - wx.x, wx.y - xe.y});
+    };
+#pragma unroll
+    for (int q = 0; q < KB / 2; ++q) {
+      const int lo = t + q * TPT;
+      const cpx<C> *A = z + (2 * q) * RL;      // butterfly lo (or 0)
+      const cpx<C> *B = z + (2 * q + 1) * RL;  // butterfly NB - lo (or NB/2)
+      if (lo != 0) {
+#pragma unroll
+        for (int j = 0; j < RL; ++j)
+          pair(lo + j * NB, A[j], B[RL - 1 - j], j == 0 ? wl[q] : cmul(wl[q], W[j * NB]), true);
+      } else {
+        // butterfly 0: bins 0 and m from Z[0], pairs (j, R - j) for j <= R/2
+        out[0] = ST::st({A[0].x + A[0].y, C(0)});
+        out[(int64_t)M * so] = ST::st({A[0].x - A[0].y, C(0)});
+#pragma unroll
+        for (int j = 1; j <= RL / 2; ++j)
+          pair(j * NB, A[j], A[RL - j], W[j * NB], 2 * j != RL);
+        // butterfly NB/2: pairs (j, R - 1 - j)
+#pragma unroll
+        for (int j = 0; j < RL / 2; ++j)
+          pair(NB / 2 + j * NB, B[j], B[RL - 1 - j], W[NB / 2 + j * NB], true);
+      }
+    }
+  };
+  if (os == 1) body(std::true_type{});
+  else body(std::false_type{});
+}
+
+template <typename S, bool UNUSED, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even_split(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using Rs = typename ST::Rs;
+  using C = typename Eng::C;
+  using Ch = typename Eng::Ch;
+  constexpr int TPT = Eng::TPT;
+  using EM = StaticEngineSplit<C, TPT, Eng::PAD, Ch, 1>;  // first stage mirrored
+  constexpr int M = Ch::N;
+  constexpr int R0 = Ch::radix(0);
+  constexpr int NB = M / R0;
+  constexpr int KB = NB / TPT;
+  const int g = threadIdx.x / TPT;
+  const int t = threadIdx.x % TPT;
+  const int64_t tr = (int64_t)blockIdx.x * TPB + g;
+  const bool valid = tr < a.count;
+  const E *X = static_cast<const E *>(a.in) + (valid ? tr * a.idist : 0);
+  Rs *y = static_cast<Rs *>(a.out) + (valid ? tr * a.odist : 0);
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(M);
+  const int64_t is = a.istride, os = a.ostride;
+  const bool pair_ok = a.flags & 1;
+  const C hw = (a.flags & 2) ? C(0.5) : C(1);  // adjoint of R2C: interior bins halved
+  const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
+  // Z'[e] from A = X[e], B = X[m-e] (both interior unless e == 0), w = W_n^e,
+  // swapped for the inverse DFT
+  auto pre = [&](cpx<C> A, cpx<C> B, cpx<C> w, bool e0) {
+    if (e0) {
+      A.y = C(0);
+      B.y = C(0);
+    } else {
+      A = {hw * A.x, hw * A.y};
+      B = {hw * B.x, hw * B.y};
+    }
+    const cpx<C> s = {A.x + B.x, A.y - B.y};     // A + conj B
+    const cpx<C> d = {A.x - B.x, A.y + B.y};     // A - conj B
+    const cpx<C> wd = cconj_mul(d, w);           // W^{-e} (A - conj B)
+    return cswap(cpx<C>{s.x - wd.y, s.y + wd.x});  // s + i wd
+  };
+  cpx<C> vin[KB * R0];
+  auto build = [&](auto unit) {
+    constexpr bool U = decltype(unit)::value;  // unit input stride
+    const int64_t si = U ? int64_t(1) : is;
+    auto ld = [&](int64_t k) { return ST::ld(X[k * si]); };
+#pragma unroll
+    for (int q = 0; q < KB / 2; ++q) {
+      const int lo = t + q * TPT;
+      const int hi = lo == 0 ? NB / 2 : NB - lo;
+      cpx<C> xl[R0], xh[R0];
+#pragma unroll
+      for (int j = 0; j < R0; ++j) {
+        xl[j] = valid ? ld(lo + j * NB) : cpx<C>{C(0), C(0)};
+        xh[j] = valid ? ld(hi + j * NB) : cpx<C>{C(0), C(0)};
+      }
+      // each pair of loaded values feeds two outputs, so the inputs can be
+      // overwritten as the outputs are formed
+      cpx<C> *vl = vin + (2 * q) * R0, *vh = vin + (2 * q + 1) * R0;
+      const cpx<C> wlo = W[lo], whi = W[hi];
+      if (lo != 0) {
+#pragma unroll
+        for (int j = 0; j < R0; ++j) {
+          const cpx<C> A = xl[j], B = xh[R0 - 1 - j];
+          vl[j] = pre(A, B, j == 0 ? wlo : cmul(wlo, W[j * NB]), false);
+          vh[R0 - 1 - j] = pre(B, A, R0 - 1 - j == 0 ? whi : cmul(whi, W[(R0 - 1 - j) * NB]),
+                               false);
+        }
+      } else {
+        const cpx<C> xm = valid ? ld(M) : cpx<C>{C(0), C(0)};  // X[m] pairs with X[0]
+        vl[0] = pre(xl[0], xm, W[0], true);
+#pragma unroll
+        for (int j = 1; j <= R0 / 2; ++j) {
+          const cpx<C> A = xl[j], B = xl[R0 - j];
+          vl[j] = pre(A, B, W[j * NB], false);
+          if (2 * j != R0) vl[R0 - j] = pre(B, A, W[(R0 - j) * NB], false);
+        }
+#pragma unroll
+        for (int j = 0; j < R0 / 2; ++j) {
+          const cpx<C> A = xh[j], B = xh[R0 - 1 - j];
+          vh[j] = pre(A, B, W[NB / 2 + j * NB], false);
+          vh[R0 - 1 - j] = pre(B, A, W[NB / 2 + (R0 - 1 - j) * NB], false);
+        }
+      }
+    }
+  };
+  if (is == 1) build(std::true_type{});
+  else build(std::false_type{});
+  EM::run_regs(
+      vin, lds, static_cast<const cpx<C> *>(a.tw), t,
+      [&](int e, cpx<C> v) {
+        v = cswap(v);
+        if (pair_ok) {
+          ST::stnt(reinterpret_cast<E *>(y) + e, v);
+        } else {
+          y[(int64_t)(2 * e) * os] = ST::str(v.x);
+          y[(int64_t)(2 * e + 1) * os] = ST::str(v.y);
+        }
+      },
+      valid);
+}
+
+// ---------------------------------------------------------------------------
 // C2R, even n = 2m. flags bit0: reals may be written in pairs.
 // aux0: W_n^k, k = 0..m-1 (forward sign; conjugated here).
 // ---------------------------------------------------------------------------

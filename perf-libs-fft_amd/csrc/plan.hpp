@@ -119,8 +119,16 @@ struct EngineData {
 // caches the winner in memory and in the wisdom file. -1 when the
 // PLFFT_KERNEL_FILTER diagnostic excludes every candidate.
 int tune_pick(int kind, int64_t n, int prec, bool bwd, const std::vector<Choice> &cands);
+// Starts the planning clock of the calling thread with its rigor.
 void set_planning_rigor(int rigor);
 int planning_rigor();
+// The reference's planning budget in seconds and decision margin per rigor
+// (src/plfft.cpp:149-184); plfft_rigor_t values.
+double planner_budget_secs(int rigor, int64_t n);
+double planner_margin(int rigor);
+// Wisdom key of a kernel choice: family, length, precision, direction and
+// the device (architecture name / CU count).
+std::string wisdom_key(int kind, int64_t n, int prec, bool bwd);
 
 // W_n^k (forward sign) for k in [0, count), interleaved re/im, FP64.
 std::vector<double> root_table(int64_t n, int64_t count);

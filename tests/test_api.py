@@ -269,3 +269,30 @@ def test_mixed_plan_without_gpu_fails_cleanly(lib):
     assert lib.plfft_mixed_plan_execute(None, None, None) == P.PLFFT_NULL_ARGUMENT
     assert lib.plfft_mixed_plan_execute_async(None, None, None, None, None) == P.PLFFT_NULL_ARGUMENT
     lib.plfft_mixed_plan_destroy(None)
+
+
+def test_planner_budget_and_margin_follow_the_reference(lib):
+    """Rigor -> planning budget (seconds) and decision margin, the reference's
+    get_planner_target_secs_total / get_planner_margin (src/plfft.cpp:149-184):
+    ESTIMATE plans without timing; n < 1500 shares one curve; MEASURE,
+    PATIENT and EXHAUSTIVE grow with n on their own curves. The library's
+    internal C++ functions are reached by their mangled names (no GPU)."""
+    budget = getattr(lib, "_ZN3plf19planner_budget_secsEil")
+    budget.restype, budget.argtypes = ctypes.c_double, [ctypes.c_int, ctypes.c_long]
+    margin = getattr(lib, "_ZN3plf14planner_marginEi")
+    margin.restype, margin.argtypes = ctypes.c_double, [ctypes.c_int]
+    # values of the reference's formulas at these lengths
+    want = {
+        (1, 64): 2.27690966431604e-05, (1, 1499): 0.000179036427437457,
+        (1, 1500): 0.00019068291013525106, (1, 4096): 0.0004982782054794112,
+        (1, 65536): 0.007060759501285816, (1, 1 << 20): 0.10005319154393118,
+        (2, 1499): 0.000179036427437457, (2, 1500): 0.00882967218749666,
+        (2, 4096): 0.017889485134077232, (2, 1 << 20): 0.8817586184232925,
+        (3, 1024): 0.00013954679201450457, (3, 1500): 0.31054855123728353,
+        (3, 65536): 2.3375085797605277, (3, 1 << 20): 10.28572555746398,
+    }
+    for (r, n), v in want.items():
+        assert budget(r, n) == pytest.approx(v, rel=1e-12), (r, n)
+    for n in (1, 64, 4096, 1 << 20):
+        assert budget(P.PLFFT_ESTIMATE, n) == 0.0
+    assert [margin(r) for r in range(4)] == [0.01, 0.02, 0.05, 0.5]

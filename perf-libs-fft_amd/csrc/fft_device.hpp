@@ -7,6 +7,8 @@
 // directions at no arithmetic cost.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -125,6 +127,26 @@ __device__ __forceinline__ cpx<T> mul_wc(cpx<T> a) {
     constexpr T s = T(-w.s);
     return cpx<T>{a.x * c - a.y * s, a.x * s + a.y * c};
   }
+}
+
+// exp(-2*pi*i*E/R) as a constant
+template <int R, int E, typename T>
+__device__ __forceinline__ cpx<T> root_c() {
+  return mul_wc<R, E>(cpx<T>{T(1), T(0)});
+}
+
+// f(std::integral_constant<int, I>{}) for I = 0 .. N-1: unrolled loops whose
+// index must be a compile-time constant (twiddle constants, codelet slots)
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for_impl(F &f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for_impl<I + 1, N>(f);
+  }
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl<0, N>(f);
 }
 
 // ---------------------------------------------------------------------------

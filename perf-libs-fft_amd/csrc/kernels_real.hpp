@@ -156,9 +156,11 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even_split(KArgs a) {
   const cpx<C> *W = static_cast<const cpx<C> *>(a.aux0);
   const C h = (a.flags & 2) ? C(1) : C(0.5);
   // post-twiddles of the low butterflies: W_n^{lo + j NB} = W_n^lo W_n^{j NB}
+  // with W_n^{NB} = exp(-2 pi i / 2RL), so only W_n^lo is loaded
   cpx<C> wl[KB / 2];
 #pragma unroll
   for (int q = 0; q < KB / 2; ++q) wl[q] = W[t + q * TPT];
+  static_assert(2 * M == 2 * RL * NB, "n = 2 RL NB");
   cpx<C> z[KB * RL];
   EM::run_keep(lds, static_cast<const cpx<C> *>(a.tw), t, valid,
                [&](int e) { return load_pair<ST>(x, e, is, pair_ok); }, z);
@@ -180,20 +182,24 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_r2c_even_split(KArgs a) {
       const cpx<C> *A = z + (2 * q) * RL;      // butterfly lo (or 0)
       const cpx<C> *B = z + (2 * q + 1) * RL;  // butterfly NB - lo (or NB/2)
       if (lo != 0) {
-#pragma unroll
-        for (int j = 0; j < RL; ++j)
-          pair(lo + j * NB, A[j], B[RL - 1 - j], j == 0 ? wl[q] : cmul(wl[q], W[j * NB]), true);
+        static_for<RL>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          pair(lo + j * NB, A[j], B[RL - 1 - j], mul_wc<2 * RL, j>(wl[q]), true);
+        });
       } else {
         // butterfly 0: bins 0 and m from Z[0], pairs (j, R - j) for j <= R/2
         out[0] = ST::st({A[0].x + A[0].y, C(0)});
         out[(int64_t)M * so] = ST::st({A[0].x - A[0].y, C(0)});
-#pragma unroll
-        for (int j = 1; j <= RL / 2; ++j)
-          pair(j * NB, A[j], A[RL - j], W[j * NB], 2 * j != RL);
+        static_for<RL / 2 + 1>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          if constexpr (j >= 1)
+            pair(j * NB, A[j], A[RL - j], root_c<2 * RL, j, C>(), 2 * j != RL);
+        });
         // butterfly NB/2: pairs (j, R - 1 - j)
-#pragma unroll
-        for (int j = 0; j < RL / 2; ++j)
-          pair(NB / 2 + j * NB, B[j], B[RL - 1 - j], W[NB / 2 + j * NB], true);
+        static_for<RL / 2>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          pair(NB / 2 + j * NB, B[j], B[RL - 1 - j], root_c<4 * RL, 2 * j + 1, C>(), true);
+        });
       }
     }
   };
@@ -250,38 +256,48 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2r_even_split(KArgs a) {
       const int lo = t + q * TPT;
       const int hi = lo == 0 ? NB / 2 : NB - lo;
       cpx<C> xl[R0], xh[R0];
+      if (valid) {
 #pragma unroll
-      for (int j = 0; j < R0; ++j) {
-        xl[j] = valid ? ld(lo + j * NB) : cpx<C>{C(0), C(0)};
-        xh[j] = valid ? ld(hi + j * NB) : cpx<C>{C(0), C(0)};
+        for (int j = 0; j < R0; ++j) {
+          xl[j] = ld(lo + j * NB);
+          xh[j] = ld(hi + j * NB);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < R0; ++j) xl[j] = xh[j] = cpx<C>{C(0), C(0)};
       }
       // each pair of loaded values feeds two outputs, so the inputs can be
       // overwritten as the outputs are formed
+      // W_n^{b + j NB} = W_n^b exp(-2 pi i j / 2R0) (n = 2 R0 NB), and
+      // W_n^{NB - lo} = exp(-2 pi i / 2R0) conj(W_n^lo): one table load per
+      // butterfly pair
       cpx<C> *vl = vin + (2 * q) * R0, *vh = vin + (2 * q + 1) * R0;
-      const cpx<C> wlo = W[lo], whi = W[hi];
       if (lo != 0) {
-#pragma unroll
-        for (int j = 0; j < R0; ++j) {
+        const cpx<C> wlo = W[lo];
+        const cpx<C> whi = mul_wc<2 * R0, 1>(cpx<C>{wlo.x, -wlo.y});
+        static_for<R0>([&](auto J) {
+          constexpr int j = decltype(J)::value;
           const cpx<C> A = xl[j], B = xh[R0 - 1 - j];
-          vl[j] = pre(A, B, j == 0 ? wlo : cmul(wlo, W[j * NB]), false);
-          vh[R0 - 1 - j] = pre(B, A, R0 - 1 - j == 0 ? whi : cmul(whi, W[(R0 - 1 - j) * NB]),
-                               false);
-        }
+          vl[j] = pre(A, B, mul_wc<2 * R0, j>(wlo), false);
+          vh[R0 - 1 - j] = pre(B, A, mul_wc<2 * R0, R0 - 1 - j>(whi), false);
+        });
       } else {
         const cpx<C> xm = valid ? ld(M) : cpx<C>{C(0), C(0)};  // X[m] pairs with X[0]
-        vl[0] = pre(xl[0], xm, W[0], true);
-#pragma unroll
-        for (int j = 1; j <= R0 / 2; ++j) {
-          const cpx<C> A = xl[j], B = xl[R0 - j];
-          vl[j] = pre(A, B, W[j * NB], false);
-          if (2 * j != R0) vl[R0 - j] = pre(B, A, W[(R0 - j) * NB], false);
-        }
-#pragma unroll
-        for (int j = 0; j < R0 / 2; ++j) {
+        vl[0] = pre(xl[0], xm, cpx<C>{C(1), C(0)}, true);
+        static_for<R0 / 2 + 1>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          if constexpr (j >= 1) {
+            const cpx<C> A = xl[j], B = xl[R0 - j];
+            vl[j] = pre(A, B, root_c<2 * R0, j, C>(), false);
+            if constexpr (2 * j != R0) vl[R0 - j] = pre(B, A, root_c<2 * R0, R0 - j, C>(), false);
+          }
+        });
+        static_for<R0 / 2>([&](auto J) {
+          constexpr int j = decltype(J)::value;
           const cpx<C> A = xh[j], B = xh[R0 - 1 - j];
-          vh[j] = pre(A, B, W[NB / 2 + j * NB], false);
-          vh[R0 - 1 - j] = pre(B, A, W[NB / 2 + (R0 - 1 - j) * NB], false);
-        }
+          vh[j] = pre(A, B, root_c<4 * R0, 2 * j + 1, C>(), false);
+          vh[R0 - 1 - j] = pre(B, A, root_c<4 * R0, 2 * (R0 - 1 - j) + 1, C>(), false);
+        });
       }
     }
   };

@@ -300,3 +300,63 @@ def test_size_limits_fail_cleanly(torch_cuda):
     with pytest.raises(P.PlfftError) as ei:
         P.Plan("r2c", (1 << 24) + 2, -1, "f64", rigor=P.PLFFT_ESTIMATE)
     assert ei.value.status == P.PLFFT_PLAN_CREATION_FAILED
+
+
+@pytest.mark.parametrize("n", [65536, 65537])
+def test_one_plan_from_four_host_threads(torch_cuda, orc, n):
+    """include/plfft.h allows concurrent execute of one plan from several
+    threads; multi-pass plans (four-step 65536, chirp-z 65537) take one
+    workspace per concurrent synchronous call from the plan's pool."""
+    import threading
+    torch = torch_cuda
+    hm = 3
+    plan = P.Plan("c2c", n, -1, "f32", howmany=hm)
+    print(f"threads: {plan.describe()[:80]}", flush=True)
+    assert plan.workspace_size > 0
+    xs = [rng.complex_uniform(40 + t, n * hm) for t in range(4)]
+    xd = [to_dev(torch, x) for x in xs]
+    yd = [torch.empty_like(d) for d in xd]
+    torch.cuda.synchronize()
+    errors = []
+
+    def work(t):
+        try:
+            for _ in range(3):
+                plan.execute(xd[t], yd[t])  # ctypes releases the GIL
+        except Exception as e:  # reported to the main thread
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errors
+    for t in range(4):
+        ref = orc.c2c_batched(xs[t].astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
+        assert relerr(from_dev(torch, yd[t]), ref) < TOL["f32"], t
+
+
+# test/plfft_may_alias_test.cpp:39-42: every size of the reference's
+# aliasing test (powers of two, primes for Rader / Bluestein / chirp-z,
+# four-step 65536)
+ALIAS_SIZES = [2, 3, 4, 5, 7, 8, 11, 16, 21, 32, 47, 64, 97, 128, 256, 509, 1024, 2053, 4096,
+               8191, 16384, 65536]
+
+
+@pytest.mark.parametrize("n", ALIAS_SIZES)
+def test_alias_sizes_of_the_reference(torch_cuda, orc, n):
+    """MAY_ALIAS with exact and +-1 element partial overlap at the
+    reference's sizes (test/plfft_may_alias_test.cpp:44-79)."""
+    torch = torch_cuda
+    x = rng.complex_uniform(0, n)
+    ref = orc.fft(x.astype(np.complex128))
+    plan = P.Plan("c2c", n, -1, "f32", may_alias=True)
+    for shift in (0, 1, -1):
+        buf = np.zeros(n + 2, np.complex64)
+        buf[1:1 + n] = x
+        d = to_dev(torch, buf)
+        base = d.data_ptr()
+        plan.execute(base + 8, base + 8 * (1 + shift))
+        out = from_dev(torch, d)[1 + shift:1 + shift + n]
+        assert relerr(out, ref) < TOL["f32"], (shift, plan.describe())

@@ -250,3 +250,23 @@ def test_r2c_c2r_fourstep_known_pairs(torch_cuda):
     want = np.zeros(m, np.complex128)
     want[k0] = n / 2
     assert np.max(np.abs(Xh - want)) < 1e-9
+
+
+@pytest.mark.parametrize("n", [16, 509, 4096, 65536])
+def test_r2c_partial_overlap(torch_cuda, orc, n):
+    """MAY_ALIAS R2C whose output starts one complex element before, at, or
+    after its input (the C2C overlap cases of test/plfft_may_alias_test.cpp
+    applied to the real transform)."""
+    torch = torch_cuda
+    x = rng.normal_f64(7, n).astype(np.float32)
+    ref = orc.r2c_batched(x.astype(np.float64), n, 1, 1, n, 1, n // 2 + 1, n // 2 + 1)
+    plan = P.Plan("r2c", n, -1, "f32", may_alias=True)
+    m = n // 2 + 1
+    for shift in (0, 1, -1):  # complex elements
+        buf = np.zeros(2 * (m + 2) + 2 * ((n + 1) // 2), np.float32)  # whole complex elements
+        buf[2:2 + n] = x  # input starts at float 2 = complex 1
+        d = to_dev(torch, buf)
+        base = d.data_ptr()
+        plan.execute(base + 8, base + 8 * (1 + shift))
+        out = from_dev(torch, d).view(np.complex64)[1 + shift:1 + shift + m]
+        assert relerr(out, ref) < TOL["f32"], (shift, plan.describe())

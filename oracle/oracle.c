@@ -589,26 +589,49 @@ int32_t orc_q_convert(double v, int bits) {
 /* fixed-point C2C (Q15 / Q7), power-of-two n                                */
 /* ------------------------------------------------------------------------ */
 /*
- * The reference's fixed-point transform (src/providers/jit/print_algo.cpp:
- * 1142-1239 input load, 2110-2196 twiddle multiply; src/providers/jit/
- * emit.hpp:150-430 saturating ops) is a Cooley-Tukey DIT over radix-2/4/8/16
- * codelets (src/providers/aot/aot_kernel_data.cpp:290-295) in which every
- * codelet first rounds-and-shifts its inputs right by log2(radix) (srshr),
- * then multiplies by the stage twiddles with one rounding per component
- * (smull/smlal + sqrshrn), then runs its butterflies with saturating
- * add/sub and rounded constant multiplies. Twiddles and constants are
- * sat(trunc(float(x) * 2^bits)) (src/plfft_convert.hpp:40-62), so 1.0 maps to
- * 2^bits - 1. The total effect is X/n.
+ * The reference's fixed-point C2C (power-of-two n, ESTIMATE plan) restated.
  *
- * This restatement fixes one concrete op graph with those primitives, which
- * is the one the GPU kernels execute: a Stockham DIT with the radix chain
- * [2^(e mod 4) if nonzero, then 16, 16, ...] (e = log2 n), stage twiddle
- * W_{LR}^{j m} skipped when j m = 0, and the codelets below. The exact bits
- * of the reference's generated codelets (WFTA / split-radix op order, IR
- * rewrites) are not reproducible here, so bit parity with the reference is
- * unpinned; this function is pinned by the reference's own fixed-point test
- * semantics instead (tests/test_oracle.py: delta <-> constant known pairs and
- * DFT/n within 6 log2(n) LSB, test/test_utils.hpp:132-151).
+ * Plan (src/planner/planner.cpp:49-182 first fit; cooley_tukey.hpp:622-748):
+ * with the fixed-point codelet sizes {2, 4, 8, 16}
+ * (src/providers/aot/aot_kernel_data.cpp:290-295), n = 16 m splits as
+ * Cooley-Tukey DIT with the radix-16 pass last, recursively, until the
+ * innermost length (2, 4, 8 or 16) is a direct codelet: passes, first to
+ * last, of radices [2^(e mod 4) if nonzero, 16, 16, ...] (e = log2 n). Pass s
+ * (radix R, L = product of the earlier radices) combines inputs h of column
+ * k: a codelet for column 0 without twiddles (ab_n), and for k > 0 with the
+ * twiddle W[(k-1)(R-1) + (h-1)] on inputs h > 0 (ab_t_dit / ac_t_dit,
+ * cooley_tukey.hpp:82-143, 329-400).
+ *
+ * Codelet (print_algo.cpp:1142-1239 load, 2110-2196 twiddle, 1479-1560
+ * store): each input is loaded and shifted right by log2(R) with rounding
+ * (srshr); on twiddled columns it is then multiplied by the twiddle with the
+ * premultiplied pair w1 = (Q(cos t), Q(sin t)), w2 = (Q(-sin t), Q(cos t)) as
+ * one widening multiply-accumulate and one rounding narrow (emit.hpp:150-300):
+ *   re = sat((x.re Q(cos t) + x.im Q(-sin t) + 2^(B-1)) >> B)
+ *   im = sat((x.re Q(sin t) + x.im Q(cos t) + 2^(B-1)) >> B),
+ * t = (dir 2 pi / (L R) * h) * k in double (generate_twiddles.hpp:99-123).
+ * Then the codelet's op list (below) computes the forward DFT_R; the
+ * backward transform stores output k at (R - k) mod R instead
+ * (print_algo.cpp:1486-1487) and uses dir = +1 twiddles.
+ *
+ * Op lists: the generator's expressions (Winograd separated forms for 2, 4,
+ * 8, winograd_z{2,4,8}_separated.cpp; split radix for 16 in fixed point,
+ * wfta_nfact.cpp:89-93, split_radix_z16.cpp:13-195), lowered as the
+ * reference emits them: '+'/'-' saturating (SQADD/SQSUB); a real constant c
+ * multiplies both lanes by Q(c) with one rounding (sqrdmulh), and c == 1 is
+ * folded away (do_opt, print_algo.cpp:564-571); an imaginary constant i s
+ * swaps the lanes and multiplies them by (Q(-s), Q(s)) (print_algo.cpp:
+ * 2230-2250); an add or subtract of two such products is one widening
+ * multiply-accumulate with one rounding narrow (emit.hpp:150-300).
+ * Q(c) = sat(trunc(float(c) 2^B)) (src/plfft_convert.hpp:40-62), so 1.0 maps
+ * to 2^B - 1 and -1.0 to -2^B.
+ *
+ * Parity unpinned: bit equality with the reference's generated code depends
+ * on do_opt rewrites (print_algo.cpp:236-973) beyond the two restated here
+ * and on the exact IR the emitter sees; no reference outputs exist to check
+ * against. This function is pinned by the reference's own fixed-point test
+ * semantics (tests/test_oracle.py: delta <-> constant, DFT/n within
+ * 6 log2(n) LSB, test/test_utils.hpp:132-151) and by planted mistakes.
  */
 typedef struct {
   int64_t re, im;
@@ -625,10 +648,6 @@ static int64_t q_sat(int64_t v) {
   const int64_t hi = (1LL << orc_q_bits_g) - 1, lo = -(1LL << orc_q_bits_g);
   return v > hi ? hi : (v < lo ? lo : v);
 }
-static qc_t q_add(qc_t a, qc_t b) { qc_t r = {q_sat(a.re + b.re), q_sat(a.im + b.im)}; return r; }
-static qc_t q_sub(qc_t a, qc_t b) { qc_t r = {q_sat(a.re - b.re), q_sat(a.im - b.im)}; return r; }
-/* (x) * (-i) = (im, -re), the negation saturating */
-static qc_t q_mul_negi(qc_t a) { qc_t r = {a.im, q_sat(-a.re)}; return r; }
 /* floor(v / 2^s) */
 static int64_t q_floor_shift(int64_t v, int s) {
   return v >= 0 ? (v >> s) : -((-v + (1LL << s) - 1) >> s);
@@ -639,76 +658,244 @@ static int64_t q_rshr(int64_t v, int s) {
   if (orc_fixed_plant_g == 1) return q_floor_shift(v, s);
   return q_floor_shift(v + (1LL << (s - 1)), s);
 }
-/* complex multiply with one rounding per component (widening MAC + rounding
- * saturating narrow by `bits`) */
-static qc_t q_cmul(qc_t x, qc_t w) {
-  const int b = orc_q_bits_g;
+/* widening product-sum of B-bit lanes kept in the 2B+2-bit accumulator the
+ * instructions use (smull/smlal: 32 bits for Q15, 16 bits for Q7; wraps) */
+static int64_t q_wrap_acc(int64_t v) {
+  const int w = 2 * orc_q_bits_g + 2;
+  const int64_t m = 1LL << w;
+  v &= m - 1;
+  return v >= (m >> 1) ? v - m : v;
+}
+/* sqrshrn / sqrdmulh rounding: sat((acc + 2^(B-1)) >> B) */
+static int64_t q_narrow(int64_t acc) {
+  return q_sat(q_floor_shift(q_wrap_acc(acc) + (1LL << (orc_q_bits_g - 1)), orc_q_bits_g));
+}
+static int64_t q_const(double c) { return orc_q_convert(c, orc_q_bits_g); }
+
+/* one codelet op: kind '+', '-', '=', 'R' (real constant), 'I' (imaginary
+ * constant); registers: inputs x_i = i, Y_k = 16 + k, locals l_k = 32 + k,
+ * named temporaries from 140, outputs y_k = 200 + k */
+typedef struct {
+  char kind;
+  short d, a, b;
+  double c;
+} fop_t;
+#define X_(i) (i)
+#define Y_(k) (16 + (k))
+#define L_(k) (32 + (k))
+#define T_(k) (140 + (k))
+#define O_(k) (200 + (k))
+#define ADD(d, a, b) {'+', d, a, b, 0}
+#define SUB(d, a, b) {'-', d, a, b, 0}
+#define CPY(d, a) {'=', d, a, 0, 0}
+#define RMUL(d, a, c) {'R', d, a, 0, c}
+#define IMUL(d, a, c) {'I', d, a, 0, c}
+
+/* winograd_z2_separated.cpp: S0 = x0 + x1, S1 = x0 - x1, multipliers 1, 1 */
+static const fop_t Q_Z2[] = {ADD(O_(0), X_(0), X_(1)), SUB(O_(1), X_(0), X_(1))};
+/* winograd_z4_separated.cpp: z4_in, multipliers [1, 1, 1, i], z4_out */
+static const fop_t Q_Z4[] = {
+    ADD(L_(0), X_(0), X_(2)), SUB(L_(1), X_(0), X_(2)), ADD(L_(2), X_(1), X_(3)),
+    SUB(L_(3), X_(1), X_(3)), ADD(L_(4), L_(0), L_(2)), SUB(L_(5), L_(0), L_(2)),
+    IMUL(T_(3), L_(3), 1.0),
+    ADD(T_(10), L_(1), T_(3)), SUB(T_(11), L_(1), T_(3)),
+    CPY(O_(0), L_(4)), CPY(O_(1), T_(11)), CPY(O_(2), L_(5)), CPY(O_(3), T_(10))};
+/* winograd_z8_separated.cpp: u = 2 pi / 8; multipliers
+ * [1, 1, 1, i sin 2u, 1, i sin 2u, i sin u, cos u] */
+#define Q_U (2.0 * 3.141592653589793 / 8.0)
+static fop_t Q_Z8[] = {
+    ADD(L_(0), X_(0), X_(4)), SUB(L_(1), X_(0), X_(4)), ADD(L_(2), X_(2), X_(6)),
+    SUB(L_(3), X_(2), X_(6)), ADD(L_(4), X_(1), X_(5)), SUB(L_(5), X_(1), X_(5)),
+    ADD(L_(6), X_(3), X_(7)), SUB(L_(7), X_(3), X_(7)), ADD(L_(8), L_(0), L_(2)),
+    SUB(L_(9), L_(0), L_(2)), ADD(L_(10), L_(4), L_(6)), SUB(L_(11), L_(4), L_(6)),
+    ADD(L_(12), L_(8), L_(10)), SUB(L_(13), L_(8), L_(10)), ADD(L_(14), L_(5), L_(7)),
+    SUB(L_(15), L_(5), L_(7)),
+    /* v = [l12, l13, l9, l11, l1, l3, l14, l15] times the multipliers
+     * (constants filled in by q_init_z8 with the generator's sin / cos) */
+    IMUL(T_(3), L_(11), 0), IMUL(T_(5), L_(3), 0), IMUL(T_(6), L_(14), 0),
+    RMUL(T_(7), L_(15), 0),
+    /* z8_out over X = [l12, l13, l9, t3, l1, t5, t6, t7] */
+    ADD(T_(20), L_(9), T_(3)), SUB(T_(21), L_(9), T_(3)), ADD(T_(22), L_(1), T_(7)),
+    SUB(T_(23), L_(1), T_(7)), ADD(T_(24), T_(5), T_(6)), SUB(T_(25), T_(5), T_(6)),
+    ADD(T_(26), T_(22), T_(24)), SUB(T_(27), T_(22), T_(24)), ADD(T_(28), T_(23), T_(25)),
+    SUB(T_(29), T_(23), T_(25)),
+    CPY(O_(0), L_(12)), CPY(O_(1), T_(27)), CPY(O_(2), T_(21)), CPY(O_(3), T_(28)),
+    CPY(O_(4), L_(13)), CPY(O_(5), T_(29)), CPY(O_(6), T_(20)), CPY(O_(7), T_(26))};
+static void q_init_z8(void) {
+  Q_Z8[16].c = sin(2.0 * Q_U);
+  Q_Z8[17].c = sin(2.0 * Q_U);
+  Q_Z8[18].c = sin(Q_U);
+  Q_Z8[19].c = cos(Q_U);
+}
+/* split_radix_z16.cpp:13-195 (Y are temporaries, then the outputs) */
+#define SR_C1 0.707106781186547572737310929369
+#define SR_C1N -0.707106781186547461715008466854
+static const fop_t Q_Z16[] = {
+    ADD(Y_(0), X_(0), X_(8)), SUB(Y_(1), X_(0), X_(8)), CPY(Y_(2), X_(4)), CPY(Y_(3), X_(12)),
+    ADD(L_(0), Y_(2), Y_(3)), SUB(L_(1), Y_(2), Y_(3)), IMUL(L_(2), L_(1), 1.0),
+    ADD(L_(3), Y_(0), L_(0)), SUB(L_(4), Y_(0), L_(0)), SUB(L_(5), Y_(1), L_(2)),
+    ADD(L_(6), Y_(1), L_(2)),
+    CPY(Y_(0), L_(3)), CPY(Y_(1), L_(5)), CPY(Y_(2), L_(4)), CPY(Y_(3), L_(6)),
+    ADD(Y_(4), X_(2), X_(10)), SUB(Y_(5), X_(2), X_(10)), ADD(Y_(6), X_(6), X_(14)),
+    SUB(Y_(7), X_(6), X_(14)),
+    ADD(L_(7), Y_(4), Y_(6)), SUB(L_(8), Y_(4), Y_(6)), IMUL(L_(9), L_(8), 1.0),
+    ADD(L_(10), Y_(0), L_(7)), SUB(L_(11), Y_(0), L_(7)), SUB(L_(12), Y_(2), L_(9)),
+    ADD(L_(13), Y_(2), L_(9)),
+    CPY(Y_(0), L_(10)), CPY(Y_(2), L_(12)), CPY(Y_(4), L_(11)), CPY(Y_(6), L_(13)),
+    RMUL(L_(14), Y_(5), SR_C1), IMUL(L_(15), L_(14), -1.0), RMUL(L_(16), Y_(7), SR_C1N),
+    IMUL(L_(17), L_(16), 1.0), ADD(L_(18), L_(14), L_(15)), ADD(L_(19), L_(16), L_(17)),
+    ADD(L_(20), L_(18), L_(19)), SUB(L_(21), L_(19), L_(18)), IMUL(L_(22), L_(21), -1.0),
+    ADD(L_(23), Y_(1), L_(20)), SUB(L_(24), Y_(1), L_(20)), SUB(L_(25), Y_(3), L_(22)),
+    ADD(L_(26), Y_(3), L_(22)),
+    CPY(Y_(1), L_(23)), CPY(Y_(3), L_(25)), CPY(Y_(5), L_(24)), CPY(Y_(7), L_(26)),
+    ADD(Y_(8), X_(1), X_(9)), SUB(Y_(9), X_(1), X_(9)), CPY(Y_(10), X_(5)), CPY(Y_(11), X_(13)),
+    ADD(L_(27), Y_(10), Y_(11)), SUB(L_(28), Y_(10), Y_(11)), IMUL(L_(29), L_(28), 1.0),
+    ADD(L_(30), Y_(8), L_(27)), SUB(L_(31), Y_(8), L_(27)), SUB(L_(32), Y_(9), L_(29)),
+    ADD(L_(33), Y_(9), L_(29)),
+    CPY(Y_(8), L_(30)), CPY(Y_(9), L_(32)), CPY(Y_(10), L_(31)), CPY(Y_(11), L_(33)),
+    ADD(Y_(12), X_(3), X_(11)), SUB(Y_(13), X_(3), X_(11)), CPY(Y_(14), X_(7)),
+    CPY(Y_(15), X_(15)),
+    ADD(L_(34), Y_(14), Y_(15)), SUB(L_(35), Y_(14), Y_(15)), IMUL(L_(36), L_(35), 1.0),
+    ADD(L_(37), Y_(12), L_(34)), SUB(L_(38), Y_(12), L_(34)), SUB(L_(39), Y_(13), L_(36)),
+    ADD(L_(40), Y_(13), L_(36)),
+    CPY(Y_(12), L_(37)), CPY(Y_(13), L_(39)), CPY(Y_(14), L_(38)), CPY(Y_(15), L_(40)),
+    ADD(L_(41), Y_(8), Y_(12)), SUB(L_(42), Y_(8), Y_(12)), IMUL(L_(43), L_(42), 1.0),
+    ADD(L_(44), Y_(0), L_(41)), SUB(L_(45), Y_(0), L_(41)), SUB(L_(46), Y_(4), L_(43)),
+    ADD(L_(47), Y_(4), L_(43)),
+    CPY(Y_(0), L_(44)), CPY(Y_(4), L_(46)), CPY(Y_(8), L_(45)), CPY(Y_(12), L_(47)),
+    RMUL(L_(48), Y_(9), 0.923879532511286738483136105060),
+    IMUL(L_(49), Y_(9), -0.382683432365089781779232680492),
+    RMUL(L_(50), Y_(13), 0.382683432365089837290383911750),
+    IMUL(L_(51), Y_(13), -0.923879532511286738483136105060),
+    ADD(L_(52), L_(48), L_(49)), ADD(L_(53), L_(50), L_(51)), ADD(L_(54), L_(52), L_(53)),
+    SUB(L_(55), L_(53), L_(52)), IMUL(L_(56), L_(55), -1.0),
+    ADD(L_(57), Y_(1), L_(54)), SUB(L_(58), Y_(1), L_(54)), SUB(L_(59), Y_(5), L_(56)),
+    ADD(L_(60), Y_(5), L_(56)),
+    CPY(Y_(1), L_(57)), CPY(Y_(5), L_(59)), CPY(Y_(9), L_(58)), CPY(Y_(13), L_(60)),
+    RMUL(L_(61), Y_(10), SR_C1), IMUL(L_(62), L_(61), -1.0), RMUL(L_(63), Y_(14), SR_C1N),
+    IMUL(L_(64), L_(63), 1.0), ADD(L_(65), L_(61), L_(62)), ADD(L_(66), L_(63), L_(64)),
+    ADD(L_(67), L_(65), L_(66)), SUB(L_(68), L_(66), L_(65)), IMUL(L_(69), L_(68), -1.0),
+    ADD(L_(70), Y_(2), L_(67)), SUB(L_(71), Y_(2), L_(67)), SUB(L_(72), Y_(6), L_(69)),
+    ADD(L_(73), Y_(6), L_(69)),
+    CPY(Y_(2), L_(70)), CPY(Y_(6), L_(72)), CPY(Y_(10), L_(71)), CPY(Y_(14), L_(73)),
+    RMUL(L_(74), Y_(11), 0.382683432365089837290383911750),
+    IMUL(L_(75), Y_(11), -0.923879532511286738483136105060),
+    RMUL(L_(76), Y_(15), -0.923879532511286849505438567576),
+    IMUL(L_(77), Y_(15), 0.382683432365089670756930217976),
+    ADD(L_(78), L_(74), L_(75)), ADD(L_(79), L_(76), L_(77)), ADD(L_(80), L_(78), L_(79)),
+    SUB(L_(81), L_(79), L_(78)), IMUL(L_(82), L_(81), -1.0),
+    ADD(L_(83), Y_(3), L_(80)), SUB(L_(84), Y_(3), L_(80)), SUB(L_(85), Y_(7), L_(82)),
+    ADD(L_(86), Y_(7), L_(82)),
+    CPY(Y_(3), L_(83)), CPY(Y_(7), L_(85)), CPY(Y_(11), L_(84)), CPY(Y_(15), L_(86)),
+    CPY(O_(0), Y_(0)), CPY(O_(1), Y_(1)), CPY(O_(2), Y_(2)), CPY(O_(3), Y_(3)),
+    CPY(O_(4), Y_(4)), CPY(O_(5), Y_(5)), CPY(O_(6), Y_(6)), CPY(O_(7), Y_(7)),
+    CPY(O_(8), Y_(8)), CPY(O_(9), Y_(9)), CPY(O_(10), Y_(10)), CPY(O_(11), Y_(11)),
+    CPY(O_(12), Y_(12)), CPY(O_(13), Y_(13)), CPY(O_(14), Y_(14)), CPY(O_(15), Y_(15))};
+
+/* How a register value was made by a constant multiply (for the fused
+ * multiply-accumulate): operand, lane swap, lane constants (Q values). */
+typedef struct {
+  int valid;
+  qc_t a;
+  int rev;
+  int64_t q0, q1;
+} qmul_t;
+
+/* Run one op list on the (already shifted and twiddled) inputs x[0..R),
+ * leaving the outputs in y[0..R). Registers are copied values; a '='
+ * keeps the source's product record, as the IR aliases the value. */
+static void q_run(const fop_t *ops, int nops, int R, const qc_t *x, qc_t *y) {
+  qc_t reg[216];
+  qmul_t mul[216];
+  memset(mul, 0, sizeof(mul));
+  for (int i = 0; i < R; ++i) reg[X_(i)] = x[i];
+  for (int i = 0; i < nops; ++i) {
+    const fop_t *o = &ops[i];
+    switch (o->kind) {
+      case '=':
+        reg[o->d] = reg[o->a];
+        mul[o->d] = mul[o->a];
+        break;
+      case 'R':
+      case 'I': {
+        const qc_t a = reg[o->a];
+        qmul_t m;
+        m.valid = 1;
+        m.a = a;
+        m.rev = o->kind == 'I';
+        m.q0 = q_const(o->kind == 'I' ? -o->c : o->c);
+        m.q1 = q_const(o->c);
+        if (o->kind == 'R' && o->c == 1.0) {  /* folded: multiply by uniform 1 */
+          reg[o->d] = a;
+          mul[o->d] = mul[o->a];
+          break;
+        }
+        const int64_t l0 = m.rev ? a.im : a.re, l1 = m.rev ? a.re : a.im;
+        reg[o->d].re = q_narrow(l0 * m.q0);
+        reg[o->d].im = q_narrow(l1 * m.q1);
+        mul[o->d] = m;
+        break;
+      }
+      default: {
+        const int sg = o->kind == '+' ? 1 : -1;
+        const qmul_t *ma = &mul[o->a], *mb = &mul[o->b];
+        qc_t r;
+        if (ma->valid && mb->valid) {  /* one accumulate, one rounding */
+          const int64_t a0 = ma->rev ? ma->a.im : ma->a.re, a1 = ma->rev ? ma->a.re : ma->a.im;
+          const int64_t b0 = mb->rev ? mb->a.im : mb->a.re, b1 = mb->rev ? mb->a.re : mb->a.im;
+          r.re = q_narrow(q_wrap_acc(a0 * ma->q0) + sg * q_wrap_acc(b0 * mb->q0));
+          r.im = q_narrow(q_wrap_acc(a1 * ma->q1) + sg * q_wrap_acc(b1 * mb->q1));
+        } else {
+          r.re = q_sat(reg[o->a].re + sg * reg[o->b].re);
+          r.im = q_sat(reg[o->a].im + sg * reg[o->b].im);
+        }
+        reg[o->d] = r;
+        mul[o->d].valid = 0;
+        break;
+      }
+    }
+  }
+  for (int k = 0; k < R; ++k) y[k] = reg[O_(k)];
+}
+
+static void q_codelet(int R, const qc_t *x, qc_t *y) {
+  switch (R) {
+    case 1: y[0] = x[0]; break;
+    case 2: q_run(Q_Z2, (int)(sizeof(Q_Z2) / sizeof(Q_Z2[0])), 2, x, y); break;
+    case 4: q_run(Q_Z4, (int)(sizeof(Q_Z4) / sizeof(Q_Z4[0])), 4, x, y); break;
+    case 8: q_run(Q_Z8, (int)(sizeof(Q_Z8) / sizeof(Q_Z8[0])), 8, x, y); break;
+    default: q_run(Q_Z16, (int)(sizeof(Q_Z16) / sizeof(Q_Z16[0])), 16, x, y); break;
+  }
+}
+
+/* DIT twiddle of input h of column k at a pass of length n = L R, the
+ * premultiplied pair of generate_twiddles.hpp, as one accumulate */
+static qc_t q_twiddle(qc_t x, int dir, int64_t n, int64_t h, int64_t k) {
+  const double base = (double)dir * 2.0 * 3.141592653589793 / (double)n;
+  const double t = base * (double)h * (double)k;
+  const double c = cos(t), s = sin(t);
+  const int64_t wr = q_const(c), wi = q_const(s), pr = q_const(-s);
   qc_t r;
-  r.re = q_sat(q_rshr(x.re * w.re - x.im * w.im, b));
-  r.im = q_sat(q_rshr(x.re * w.im + x.im * w.re, b));
+  r.re = q_narrow(q_wrap_acc(x.re * wr) + q_wrap_acc(x.im * pr));
+  r.im = q_narrow(q_wrap_acc(x.re * wi) + q_wrap_acc(x.im * wr));
   return r;
 }
 
-/* Q value of exp(-2 pi i num/den): (q(cos), q(-sin)) */
-static qc_t q_root(int64_t num, int64_t den) {
-  double s, c;
-  orc_sincos_2pi_frac(num, den, &s, &c);
-  qc_t r = {orc_q_convert(c, orc_q_bits_g), orc_q_convert(-s, orc_q_bits_g)};
-  return r;
-}
-
-static void q_dft4(qc_t *a) {
-  qc_t t0 = q_add(a[0], a[2]), t1 = q_sub(a[0], a[2]);
-  qc_t t2 = q_add(a[1], a[3]), t3 = q_mul_negi(q_sub(a[1], a[3]));
-  a[0] = q_add(t0, t2);
-  a[2] = q_sub(t0, t2);
-  a[1] = q_add(t1, t3);
-  a[3] = q_sub(t1, t3);
-}
-
-/* radix-R butterfly on inputs already shifted; w16[k] = Q(W_16^k) */
-static void q_dft(int R, qc_t *a, const qc_t *w16) {
-  if (R == 1) return;
-  if (R == 2) {
-    qc_t x = q_add(a[0], a[1]), y = q_sub(a[0], a[1]);
-    a[0] = x;
-    a[1] = y;
-    return;
+int orc_fixed_codelet(int R, int bits, const int32_t *x, int32_t *y) {
+  if ((R != 2 && R != 4 && R != 8 && R != 16) || (bits != 15 && bits != 7)) return -1;
+  orc_q_bits_g = bits;
+  q_init_z8();
+  qc_t v[16] = {{0, 0}}, o[16] = {{0, 0}};
+  for (int m = 0; m < R; ++m) {
+    v[m].re = x[2 * m];
+    v[m].im = x[2 * m + 1];
   }
-  if (R == 4) {
-    q_dft4(a);
-    return;
+  q_codelet(R, v, o);
+  for (int m = 0; m < R; ++m) {
+    y[2 * m] = (int32_t)o[m].re;
+    y[2 * m + 1] = (int32_t)o[m].im;
   }
-  if (R == 8) {
-    /* two radix-4 on even/odd inputs, internal twiddles W_8^k, radix-2 */
-    qc_t e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
-    q_dft4(e);
-    q_dft4(o);
-    o[1] = q_cmul(o[1], w16[2]);
-    o[2] = q_mul_negi(o[2]);
-    o[3] = q_cmul(o[3], w16[6]);
-    for (int k = 0; k < 4; ++k) {
-      a[k] = q_add(e[k], o[k]);
-      a[k + 4] = q_sub(e[k], o[k]);
-    }
-    return;
-  }
-  /* R == 16: radix-4 over j (stride 4), internal W_16^{j k1}, radix-4 over k1 */
-  qc_t f[4][4];
-  for (int j = 0; j < 4; ++j) {
-    for (int m = 0; m < 4; ++m) f[j][m] = a[j + 4 * m];
-    q_dft4(f[j]);
-  }
-  for (int j = 1; j < 4; ++j)
-    for (int k1 = 1; k1 < 4; ++k1) {
-      const int e = j * k1;
-      f[j][k1] = e == 4 ? q_mul_negi(f[j][k1]) : q_cmul(f[j][k1], w16[e]);
-    }
-  for (int k1 = 0; k1 < 4; ++k1) {
-    qc_t g[4] = {f[0][k1], f[1][k1], f[2][k1], f[3][k1]};
-    q_dft4(g);
-    for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = g[k2];
-  }
+  return 0;
 }
 
 int orc_fixed_chain(int64_t n, int *chain) {
@@ -726,15 +913,14 @@ int orc_fixed_c2c(int64_t n, int dir, int bits, const int32_t *x, int32_t *y) {
   const int ns = orc_fixed_chain(n, chain);
   if (ns < 0 || (bits != 15 && bits != 7)) return -1;
   orc_q_bits_g = bits;
+  q_init_z8();
   qc_t *a = (qc_t *)malloc(sizeof(qc_t) * (size_t)n);
   qc_t *b = (qc_t *)malloc(sizeof(qc_t) * (size_t)n);
-  qc_t w16[16];
-  for (int k = 0; k < 16; ++k) w16[k] = q_root(k, 16);
-  /* backward = swap o forward o swap */
   for (int64_t k = 0; k < n; ++k) {
-    a[k].re = dir > 0 ? x[2 * k + 1] : x[2 * k];
-    a[k].im = dir > 0 ? x[2 * k] : x[2 * k + 1];
+    a[k].re = x[2 * k];
+    a[k].im = x[2 * k + 1];
   }
+  const int tdir = orc_fixed_plant_g == 2 ? -dir : dir;
   int64_t L = 1;
   for (int s = 0; s < ns; ++s) {
     const int R = chain[s];
@@ -743,17 +929,16 @@ int orc_fixed_c2c(int64_t n, int dir, int bits, const int32_t *x, int32_t *y) {
     const int64_t NB = n / R;
     for (int64_t bb = 0; bb < NB; ++bb) {
       const int64_t j = bb % L;
-      qc_t v[16];
+      qc_t v[16], o[16];
       for (int m = 0; m < R; ++m) {
         v[m].re = q_rshr(a[bb + m * NB].re, sh);
         v[m].im = q_rshr(a[bb + m * NB].im, sh);
+        if (j > 0 && m > 0) v[m] = q_twiddle(v[m], tdir, L * R, m, j);
       }
-      if (j > 0)
-        for (int m = 1; m < R; ++m)
-          v[m] = q_cmul(v[m], q_root(orc_fixed_plant_g == 2 ? -j * m : j * m, L * R));
-      q_dft(R, v, w16);
+      q_codelet(R, v, o);
       const int64_t base = (bb / L) * L * R + j;
-      for (int m = 0; m < R; ++m) b[base + m * L] = v[m];
+      /* backward: output k of the forward codelet goes to (R - k) mod R */
+      for (int m = 0; m < R; ++m) b[base + (dir > 0 ? (R - m) % R : m) * L] = o[m];
     }
     qc_t *t = a;
     a = b;
@@ -761,8 +946,8 @@ int orc_fixed_c2c(int64_t n, int dir, int bits, const int32_t *x, int32_t *y) {
     L *= R;
   }
   for (int64_t k = 0; k < n; ++k) {
-    y[2 * k] = (int32_t)(dir > 0 ? a[k].im : a[k].re);
-    y[2 * k + 1] = (int32_t)(dir > 0 ? a[k].re : a[k].im);
+    y[2 * k] = (int32_t)a[k].re;
+    y[2 * k + 1] = (int32_t)a[k].im;
   }
   free(a);
   free(b);

@@ -65,6 +65,10 @@ int32_t orc_q_convert(double v, int bits);
 /* Radix chain of the fixed-point transform of n (power of two); returns the
  * number of stages, or -1 if n is not a power of two. */
 int orc_fixed_chain(int64_t n, int *chain);
+/* One forward radix-R codelet (R = 2, 4, 8, 16) of the fixed-point
+ * transform on R pairs already shifted and twiddled; returns 0, or -1 for
+ * an unsupported R or bits. */
+int orc_fixed_codelet(int R, int bits, const int32_t *x, int32_t *y);
 /* Fixed-point C2C of n = 2^e points, bits 15 (Q0.15) or 7 (Q0.7); x and y
  * hold interleaved values in int32 (within the storage range). dir -1
  * forward, +1 backward. Output = DFT(x)/n rounded stage by stage. */

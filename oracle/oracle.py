@@ -67,6 +67,9 @@ def lib() -> ctypes.CDLL:
         L.orc_fixed_c2c.argtypes = [_i64, ctypes.c_int, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.orc_fixed_c2c.restype = ctypes.c_int
+        L.orc_fixed_codelet.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_int32)]
+        L.orc_fixed_codelet.restype = ctypes.c_int
         L.orc_fixed_set_plant.argtypes = [ctypes.c_int]
         L.orc_fixed_set_plant.restype = None
         _lib = L
@@ -227,6 +230,17 @@ def fixed_c2c(x, direction: int = -1, bits: int = 15) -> np.ndarray:
     if st != 0:
         raise ValueError(f"fixed-point C2C needs a power-of-two n and bits 15 or 7 (n={n})")
     return y.reshape(n, 2)
+
+
+def fixed_codelet(x, bits: int = 15) -> np.ndarray:
+    """One forward fixed-point codelet (oracle.c orc_fixed_codelet) on R
+    (re, im) pairs, R in (2, 4, 8, 16)."""
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int32).reshape(-1, 2))
+    y = np.zeros_like(a)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    if lib().orc_fixed_codelet(a.shape[0], bits, a.ctypes.data_as(ip), y.ctypes.data_as(ip)):
+        raise ValueError(f"no fixed-point codelet of size {a.shape[0]} for {bits} bits")
+    return y
 
 
 def fixed_set_plant(mode: int) -> None:

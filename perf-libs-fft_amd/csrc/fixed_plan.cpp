@@ -32,18 +32,24 @@ int32_t q_convert(double v, int bits) {
   return (int32_t)std::min(hi, std::max(lo, t));
 }
 
-// Q value of exp(-2 pi i num/den), appended as a storage pair.
-void push_root(std::vector<int8_t> &tab, int bits, int64_t num, int64_t den) {
-  double re, im;
-  unit_root(num, den, &re, &im);
-  const int32_t qr = q_convert(re, bits), qi = q_convert(-im, bits);
-  if (bits == 15) {
-    const int16_t v[2] = {(int16_t)qr, (int16_t)qi};
-    const int8_t *b = reinterpret_cast<const int8_t *>(v);
-    tab.insert(tab.end(), b, b + 4);
-  } else {
-    tab.push_back((int8_t)qr);
-    tab.push_back((int8_t)qi);
+// The stage twiddle of input h of column k at a pass of length n, as the
+// premultiplied pairs (Q cos t, Q(-sin t)), (Q sin t, Q cos t) with
+// t = ((dir 2 pi / n) h) k evaluated in double in that order
+// (src/generate_twiddles.hpp:99-123), appended in storage pairs.
+void push_tw(std::vector<int8_t> &tab, int bits, int dir, int64_t n, int64_t h, int64_t k) {
+  const double base = (double)dir * 2.0 * 3.141592653589793 / (double)n;
+  const double t = base * (double)h * (double)k;
+  const double c = std::cos(t), si = std::sin(t);
+  const int32_t q[4] = {q_convert(c, bits), q_convert(-si, bits), q_convert(si, bits),
+                        q_convert(c, bits)};
+  for (int i = 0; i < 4; ++i) {
+    if (bits == 15) {
+      const int16_t v = (int16_t)q[i];
+      const int8_t *b = reinterpret_cast<const int8_t *>(&v);
+      tab.insert(tab.end(), b, b + 2);
+    } else {
+      tab.push_back((int8_t)q[i]);
+    }
   }
 }
 
@@ -75,10 +81,10 @@ class FixedC2CNode : public Node {
     GenericDesc d{};
     d.n = (int)n_;
     d.ns = (int)chain.size();
-    // table: 16 codelet constants Q(W_16^k), then the stage twiddles
-    // Q(W_{LR}^{j m}) at (m-1) L + j of each stage s >= 1
+    // table: the stage twiddles of input m of column j at (m-1) L + j of
+    // each stage s >= 1, for this direction (dir = +1 backward)
     std::vector<int8_t> tab;
-    for (int k = 0; k < 16; ++k) push_root(tab, bits_, k, 16);
+    const int dir = bwd_ ? 1 : -1;
     int64_t L = 1, off = 0;
     for (int s = 0; s < d.ns; ++s) {
       const int R = chain[s];
@@ -87,13 +93,13 @@ class FixedC2CNode : public Node {
       d.twoff[s] = (int)off;
       if (s > 0) {
         for (int m = 1; m < R; ++m)
-          for (int64_t j = 0; j < L; ++j) push_root(tab, bits_, j * m, L * R);
+          for (int64_t j = 0; j < L; ++j) push_tw(tab, bits_, dir, L * R, m, j);
         off += L * (R - 1);
       }
       L *= R;
     }
     hipError_t err = desc_.upload(&d, sizeof(d));
-    if (err == hipSuccess) err = tab_.upload(tab.data(), tab.size());
+    if (err == hipSuccess && !tab.empty()) err = tab_.upload(tab.data(), tab.size());
     return err;
   }
 

@@ -18,16 +18,23 @@ hipError_t launch_q(const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int BITS>
-hipError_t launch_tier(int tier, const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
+template <int BITS, bool BWD>
+hipError_t launch_dir(int tier, const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
   switch (tier) {
-    case 0: return launch_q<k_fixed<BITS, 16, 16, 16>, 256>(a, grid, lds, s);
-    case 1: return launch_q<k_fixed<BITS, 64, 4, 16>, 256>(a, grid, lds, s);
-    case 2: return launch_q<k_fixed<BITS, 256, 1, 16>, 256>(a, grid, lds, s);
-    case 3: return launch_q<k_fixed<BITS, 1024, 1, 16>, 1024>(a, grid, lds, s);
-    case 4: return launch_q<k_fixed<BITS, 1024, 1, 32>, 1024>(a, grid, lds, s);
+    case 0: return launch_q<k_fixed<BITS, 16, 16, 16, BWD>, 256>(a, grid, lds, s);
+    case 1: return launch_q<k_fixed<BITS, 64, 4, 16, BWD>, 256>(a, grid, lds, s);
+    case 2: return launch_q<k_fixed<BITS, 256, 1, 16, BWD>, 256>(a, grid, lds, s);
+    case 3: return launch_q<k_fixed<BITS, 1024, 1, 16, BWD>, 1024>(a, grid, lds, s);
+    case 4: return launch_q<k_fixed<BITS, 1024, 1, 32, BWD>, 1024>(a, grid, lds, s);
     default: return hipErrorInvalidValue;
   }
+}
+
+// flags bit0 selects the backward instances
+template <int BITS>
+hipError_t launch_tier(int tier, const KArgs &a, int64_t grid, size_t lds, hipStream_t s) {
+  return (a.flags & 1) ? launch_dir<BITS, true>(tier, a, grid, lds, s)
+                       : launch_dir<BITS, false>(tier, a, grid, lds, s);
 }
 
 template <int BITS>

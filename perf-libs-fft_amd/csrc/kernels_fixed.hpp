@@ -1,19 +1,24 @@
 // kernels_fixed.hpp -- Q0.15 / Q0.7 fixed-point C2C for power-of-two n.
 //
-// Semantics of the reference's fixed-point codelets (src/providers/jit/
-// print_algo.cpp:1142-1239, 2110-2196; src/providers/jit/emit.hpp:150-430):
-// every radix-R codelet first shifts its inputs right by log2(R) with
-// rounding (srshr), multiplies by the stage twiddles with one rounding per
-// component (widening multiply-accumulate, rounding saturating narrow),
-// then runs its butterflies with saturating add/sub; the transform is X/n.
-// Twiddles and internal constants are sat(trunc(float(x) * 2^bits)).
-//
-// The op graph fixed here: a Stockham DIT over the chain [2^(e mod 4) if
-// nonzero, then 16, 16, ...], stage twiddle W_{LR}^{j m} skipped when
-// j m = 0, radix 8 = two radix-4 + W_8 + radix-2, radix 16 = radix-4 over
-// stride-4 groups + W_16^{j k} + radix-4. Backward = swap o forward o swap,
-// exact in integers. All arithmetic fits int32: a twiddle product sees
-// inputs already shifted right by at least one bit.
+// The reference's fixed-point codelets (src/providers/jit/print_algo.cpp:
+// 1142-1239 load, 2110-2196 twiddle, 1479-1560 store; src/providers/jit/
+// emit.hpp:150-300) computed by a Stockham DIT over the chain
+// [2^(e mod 4) if nonzero, then 16, 16, ...] (e = log2 n), one radix-R
+// pass per stage:
+//   1. each input is shifted right by log2 R with rounding (srshr);
+//   2. on a twiddled column (j > 0) input m > 0 is multiplied by the stage
+//      twiddle as one widening multiply-accumulate and one rounding narrow,
+//      with the premultiplied pairs (Q cos t, Q(-sin t)) and
+//      (Q sin t, Q cos t), t = ((dir 2 pi / (L R)) m) j;
+//   3. the codelet's op list (fixed_codelets.hpp: the generator's Winograd
+//      forms for 2, 4, 8 and split radix for 16) computes the forward DFT_R
+//      with saturating adds, per-lane rounding multiplies and one-rounding
+//      multiply-accumulates;
+//   4. the backward transform stores codelet output k at slot (R - k) mod R
+//      (print_algo.cpp:1486-1487) and uses the dir = +1 twiddles.
+// The multiply-accumulates keep the 2B+2-bit accumulator of the widening
+// instructions (32 bits for Q15, 16 for Q7, wrapping) and narrow with
+// sat(floor((acc + 2^(B-1)) / 2^B)).
 #pragma once
 
 #include <stdint.h>
@@ -21,6 +26,7 @@
 #include <type_traits>
 
 #include "fft_device.hpp"
+#include "fixed_codelets.hpp"
 #include "kernel_args.hpp"
 
 namespace plf {
@@ -36,11 +42,18 @@ struct QPair<7> {
   int8_t re, im;
 };
 
+// One stage twiddle as its two premultiplied pairs: re = x . a, im = x . b
+// with a = (Q cos t, Q(-sin t)), b = (Q sin t, Q cos t).
+template <int BITS>
+struct QTw {
+  QPair<BITS> a, b;
+};
+
 struct qc {
   int re, im;
 };
 
-// Scalar form: each component widened to int32, every result clamped.
+// Scalar form: each component widened to int32.
 template <int BITS>
 struct QOps {
   static constexpr int HI = (1 << BITS) - 1, LO = -(1 << BITS);
@@ -55,34 +68,57 @@ struct QOps {
     p.im = (decltype(p.im))v.im;
     return p;
   }
-  static __device__ __forceinline__ V swap(V v) { return {v.im, v.re}; }
   static __device__ __forceinline__ V add(V a, V b) { return {sat(a.re + b.re), sat(a.im + b.im)}; }
   static __device__ __forceinline__ V sub(V a, V b) { return {sat(a.re - b.re), sat(a.im - b.im)}; }
-  // x * (-i) = (im, -re); the negation saturates (-LO > HI)
-  static __device__ __forceinline__ V negi(V a) { return {a.im, sat(-a.re)}; }
-  // srshr: (v + 2^(s-1)) >> s, arithmetic (floor), per component
+  // srshr: floor((v + 2^(S-1)) / 2^S) per component
   template <int S>
   static __device__ __forceinline__ V shr(V v) {
     if constexpr (S == 0) return v;
     else return {(v.re + (1 << (S - 1))) >> S, (v.im + (1 << (S - 1))) >> S};
   }
-  // complex product, one rounding per component
-  static __device__ __forceinline__ V cmul(V x, P wp) {
-    constexpr int H = 1 << (BITS - 1);
-    const V w = in(wp);
-    return {sat((x.re * w.re - x.im * w.im + H) >> BITS),
-            sat((x.re * w.im + x.im * w.re + H) >> BITS)};
+  // the accumulator of the widening instructions: 2 BITS + 2 bits, wrapping
+  static __device__ __forceinline__ int wrap(unsigned acc) {
+    if constexpr (BITS == 7) return (int)(int16_t)(uint16_t)acc;
+    else return (int)acc;
+  }
+  // rounding saturating narrow of a wrapped accumulator; floor of
+  // (w + 2^(B-1)) / 2^B is (w >> B) plus bit B-1 of w, with no overflow
+  static __device__ __forceinline__ int narrow(unsigned acc) {
+    const int w = wrap(acc);
+    return sat((w >> BITS) + ((w >> (BITS - 1)) & 1));
+  }
+  template <bool REV>
+  static __device__ __forceinline__ V mul(V a, int q0, int q1) {
+    const int l0 = REV ? a.im : a.re, l1 = REV ? a.re : a.im;
+    return {narrow((unsigned)(l0 * q0)), narrow((unsigned)(l1 * q1))};
+  }
+  template <int SG, bool RA, bool RB>
+  static __device__ __forceinline__ V mac(V a, int qa0, int qa1, V b, int qb0, int qb1) {
+    const int a0 = RA ? a.im : a.re, a1 = RA ? a.re : a.im;
+    const int b0 = RB ? b.im : b.re, b1 = RB ? b.re : b.im;
+    const unsigned pa0 = (unsigned)(a0 * qa0), pa1 = (unsigned)(a1 * qa1);
+    const unsigned pb0 = (unsigned)(b0 * qb0), pb1 = (unsigned)(b1 * qb1);
+    return SG > 0 ? V{narrow(pa0 + pb0), narrow(pa1 + pb1)}
+                  : V{narrow(pa0 - pb0), narrow(pa1 - pb1)};
+  }
+  static __device__ __forceinline__ V twiddle(V x, const QTw<BITS> &w) {
+    return {narrow((unsigned)(x.re * (int)w.a.re + x.im * (int)w.a.im)),
+            narrow((unsigned)(x.re * (int)w.b.re + x.im * (int)w.b.im))};
   }
 };
 
 // Packed Q0.15 form: (re, im) in one 32-bit register as two int16 lanes,
 // the same results bit for bit.
-//   add/sub   v_pk_add_i16 / v_pk_sub_i16 with clamp (saturating)
+//   add/sub   saturating packed 16-bit add / subtract
 //   shr<S>    floor(v / 2^S) + bit S-1 of v == floor((v + 2^(S-1)) / 2^S)
-//   cmul      two v_dot2 (exact int32 sums of int16 products) + rounding
-//             shift + saturating pack. Every product operand x has been
-//             shifted right by at least one bit first (|x| <= 2^14), so the
-//             16-bit negation of x.im is exact.
+//   products  v_dot2 (int16 x int16 products summed exactly into an int32
+//             accumulator, modulo 2^32 like the 32-bit accumulator of the
+//             widening instructions) with the rounding term 2^14 as its
+//             addend, then >> 15 and saturation. Adding 2^14 before the
+//             shift cannot change a result: a sum of two int16 x int16
+//             products never lies in [2^31 - 2^14, 2^31) (the only product
+//             above 2^30 - 2^15 is (-2^15)^2 = 2^30), so the wrapped sum plus
+//             2^14 stays below 2^31.
 typedef short q15x2 __attribute__((ext_vector_type(2)));
 
 struct QOps15P {
@@ -91,13 +127,8 @@ struct QOps15P {
   static __device__ __forceinline__ V zero() { return V{0, 0}; }
   static __device__ __forceinline__ V in(P p) { return __builtin_bit_cast(V, p); }
   static __device__ __forceinline__ P out(V v) { return __builtin_bit_cast(P, v); }
-  static __device__ __forceinline__ V swap(V v) { return __builtin_shufflevector(v, v, 1, 0); }
   static __device__ __forceinline__ V add(V a, V b) { return __builtin_elementwise_add_sat(a, b); }
   static __device__ __forceinline__ V sub(V a, V b) { return __builtin_elementwise_sub_sat(a, b); }
-  static __device__ __forceinline__ V negi(V a) {
-    const V n = __builtin_elementwise_sub_sat(V{0, 0}, a);
-    return __builtin_shufflevector(a, n, 1, 2);  // (a.im, sat(-a.re))
-  }
   template <int S>
   static __device__ __forceinline__ V shr(V v) {
     if constexpr (S == 0) {
@@ -108,14 +139,37 @@ struct QOps15P {
       return q + r;
     }
   }
-  static __device__ __forceinline__ V cmul(V x, P wp) {
-    const V w = in(wp);
-    const V xn = {x.x, (short)(-x.y)};
-    const int re = __builtin_amdgcn_sdot2(xn, w, 1 << 14, false) >> 15;
-    const int im = __builtin_amdgcn_sdot2(x, swap(w), 1 << 14, false) >> 15;
-    const int lo = re > 32767 ? 32767 : (re < -32768 ? -32768 : re);
-    const int hi = im > 32767 ? 32767 : (im < -32768 ? -32768 : im);
-    return V{(short)lo, (short)hi};
+  static __device__ __forceinline__ int sat(int v) {
+    return v > 32767 ? 32767 : (v < -32768 ? -32768 : v);
+  }
+  static __device__ __forceinline__ V pack(int re_acc, int im_acc) {
+    return V{(short)sat(re_acc >> 15), (short)sat(im_acc >> 15)};
+  }
+  static __device__ __forceinline__ int dot(V x, V w, int c) {
+    return __builtin_amdgcn_sdot2(x, w, c, false);
+  }
+  // constant pair selecting lane 0 (or lane 1 when REV) of the operand
+  template <bool REV>
+  static __device__ __forceinline__ V lane0(int q) {
+    return REV ? V{0, (short)q} : V{(short)q, 0};
+  }
+  template <bool REV>
+  static __device__ __forceinline__ V lane1(int q) {
+    return REV ? V{(short)q, 0} : V{0, (short)q};
+  }
+  template <bool REV>
+  static __device__ __forceinline__ V mul(V a, int q0, int q1) {
+    return pack(dot(a, lane0<REV>(q0), 1 << 14), dot(a, lane1<REV>(q1), 1 << 14));
+  }
+  template <int SG, bool RA, bool RB>
+  static __device__ __forceinline__ V mac(V a, int qa0, int qa1, V b, int qb0, int qb1) {
+    const int pb0 = dot(b, lane0<RB>(qb0), 0), pb1 = dot(b, lane1<RB>(qb1), 0);
+    const int c0 = SG > 0 ? (1 << 14) + pb0 : (1 << 14) - pb0;
+    const int c1 = SG > 0 ? (1 << 14) + pb1 : (1 << 14) - pb1;
+    return pack(dot(a, lane0<RA>(qa0), c0), dot(a, lane1<RA>(qa1), c1));
+  }
+  static __device__ __forceinline__ V twiddle(V x, const QTw<15> &w) {
+    return pack(dot(x, in(w.a), 1 << 14), dot(x, in(w.b), 1 << 14));
   }
 };
 
@@ -128,80 +182,27 @@ struct QSel<15> {
   using type = QOps15P;
 };
 
-// Radix-2/4/8/16 butterflies on shifted inputs over either form;
-// w16[k] = Q(W_16^k).
-template <class O>
-struct QBfly {
-  using V = typename O::V;
-  using P = typename O::P;
-  static __device__ __forceinline__ void dft4(V *a) {
-    const V t0 = O::add(a[0], a[2]), t1 = O::sub(a[0], a[2]);
-    const V t2 = O::add(a[1], a[3]), t3 = O::negi(O::sub(a[1], a[3]));
-    a[0] = O::add(t0, t2);
-    a[2] = O::sub(t0, t2);
-    a[1] = O::add(t1, t3);
-    a[3] = O::sub(t1, t3);
-  }
-
-  template <int R>
-  static __device__ __forceinline__ void dft(V *a, const P *w16) {
-    if constexpr (R == 2) {
-      const V x = O::add(a[0], a[1]), y = O::sub(a[0], a[1]);
-      a[0] = x;
-      a[1] = y;
-    } else if constexpr (R == 4) {
-      dft4(a);
-    } else if constexpr (R == 8) {
-      V e[4] = {a[0], a[2], a[4], a[6]}, o[4] = {a[1], a[3], a[5], a[7]};
-      dft4(e);
-      dft4(o);
-      o[1] = O::cmul(o[1], w16[2]);
-      o[2] = O::negi(o[2]);
-      o[3] = O::cmul(o[3], w16[6]);
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        a[k] = O::add(e[k], o[k]);
-        a[k + 4] = O::sub(e[k], o[k]);
-      }
-    } else if constexpr (R == 16) {
-      V f[4][4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-#pragma unroll
-        for (int m = 0; m < 4; ++m) f[j][m] = a[j + 4 * m];
-        dft4(f[j]);
-      }
-#pragma unroll
-      for (int j = 1; j < 4; ++j)
-#pragma unroll
-        for (int k1 = 1; k1 < 4; ++k1) {
-          const int e = j * k1;
-          f[j][k1] = e == 4 ? O::negi(f[j][k1]) : O::cmul(f[j][k1], w16[e]);
-        }
-#pragma unroll
-      for (int k1 = 0; k1 < 4; ++k1) {
-        V g[4] = {f[0][k1], f[1][k1], f[2][k1], f[3][k1]};
-        dft4(g);
-#pragma unroll
-        for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = g[k2];
-      }
-    }
-  }
-};
+template <class O, int BITS, int R>
+__device__ __forceinline__ void fx_dft(typename O::V *a) {
+  if constexpr (R == 2) fx_dft2<O, BITS>(a);
+  else if constexpr (R == 4) fx_dft4<O, BITS>(a);
+  else if constexpr (R == 8) fx_dft8<O, BITS>(a);
+  else fx_dft16<O, BITS>(a);
+}
 
 // Runtime-chain Stockham over radices 2, 4, 8, 16 with LDS in the storage
 // pair type. Stage 0 reads global memory, the last stage writes it.
-template <int BITS, int TPT, int PTS>
+template <int BITS, int TPT, int PTS, bool BWD>
 struct FixedEngine {
   using O = typename QSel<BITS>::type;
   using V = typename O::V;
   using P = QPair<BITS>;
+  using TW = QTw<BITS>;
   static constexpr int PAD = 5;
 
   template <int R, class Load, class Store>
   static __device__ __forceinline__ void stage(const GenericDesc *__restrict__ d, int s, P *lds,
-                                               const P *__restrict__ tw,
-                                               const P *__restrict__ w16, int t, bool valid,
+                                               const TW *__restrict__ tw, int t, bool valid,
                                                Load &load, Store &store) {
     constexpr int KMAX = (PTS + R - 1) / R;
     constexpr int SH = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
@@ -212,7 +213,7 @@ struct FixedEngine {
     const int lgL = __builtin_ctz(L);
     const bool first = s == 0;
     const bool last = s == d->ns - 1;
-    const P *tws = tw + d->twoff[s];
+    const TW *tws = tw + d->twoff[s];
     V v[KMAX][R];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -236,25 +237,29 @@ struct FixedEngine {
       const int j = b & (L - 1);
       if (!first && b < NB && j > 0) {
 #pragma unroll
-        for (int m = 1; m < R; ++m) v[k][m] = O::cmul(v[k][m], tws[(m - 1) * L + j]);
+        for (int m = 1; m < R; ++m) v[k][m] = O::twiddle(v[k][m], tws[(m - 1) * L + j]);
       }
-      QBfly<O>::template dft<R>(v[k], w16);
+      fx_dft<O, BITS, R>(v[k]);
     }
     if (!first && !last) __syncthreads();
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
       const int b = t + k * TPT;
       if (b < NB) {
+        // backward: slot m takes codelet output (R - m) mod R
+        V o[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) o[m] = v[k][BWD ? (R - m) & (R - 1) : m];
         if (last) {
           if (valid) {
 #pragma unroll
-            for (int m = 0; m < R; ++m) store(b + m * NB, v[k][m]);
+            for (int m = 0; m < R; ++m) store(b + m * NB, o[m]);
           }
         } else {
           const int j = b & (L - 1);
           const int base = ((b >> lgL) << lgL) * R + j;
 #pragma unroll
-          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = O::out(v[k][m]);
+          for (int m = 0; m < R; ++m) lds[lds_pad<PAD>(base + m * L)] = O::out(o[m]);
         }
       }
     }
@@ -263,7 +268,7 @@ struct FixedEngine {
 
   template <class Load, class Store>
   static __device__ __forceinline__ void run(const GenericDesc *__restrict__ d, P *lds,
-                                             const P *tw, const P *w16, int t, bool valid,
+                                             const TW *tw, int t, bool valid,
                                              Load &&load, Store &&store) {
     const int ns = d->ns;
     if (ns == 0) {  // n == 1: identity
@@ -272,22 +277,22 @@ struct FixedEngine {
     }
     for (int s = 0; s < ns; ++s) {
       switch (d->radix[s]) {
-        case 2: stage<2>(d, s, lds, tw, w16, t, valid, load, store); break;
-        case 4: stage<4>(d, s, lds, tw, w16, t, valid, load, store); break;
-        case 8: stage<8>(d, s, lds, tw, w16, t, valid, load, store); break;
-        case 16: stage<16>(d, s, lds, tw, w16, t, valid, load, store); break;
+        case 2: stage<2>(d, s, lds, tw, t, valid, load, store); break;
+        case 4: stage<4>(d, s, lds, tw, t, valid, load, store); break;
+        case 8: stage<8>(d, s, lds, tw, t, valid, load, store); break;
+        case 16: stage<16>(d, s, lds, tw, t, valid, load, store); break;
         default: break;
       }
     }
   }
 };
 
-// Batched fixed-point C2C. KArgs: in/out (storage pairs), tw = 16 codelet
-// constants Q(W_16^k) followed by the stage twiddles, desc, count, strides
-// and distances in complex elements, n, flags bit0 = backward; flags bit1:
+// Batched fixed-point C2C (BWD: the backward transform). KArgs: in/out
+// (storage pairs), tw = the stage twiddles (QTw, for the plan's direction),
+// desc, count, strides and distances in complex elements, n; flags bit1:
 // member offsets (complex elements) from the tables aux0 (input) and aux1
 // (output) instead of member * distance.
-template <int BITS, int TPT, int TPB, int PTS>
+template <int BITS, int TPT, int TPB, int PTS, bool BWD>
 __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   using P = QPair<BITS>;
   extern __shared__ unsigned char plf_dyn_lds[];
@@ -295,7 +300,7 @@ __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   const int64_t member = (int64_t)blockIdx.x * TPB + tb;
   const bool valid = member < a.count;
   P *lds = reinterpret_cast<P *>(plf_dyn_lds) + tb * lds_padded<5>((int)a.n);
-  const P *tab = static_cast<const P *>(a.tw);
+  const QTw<BITS> *tab = static_cast<const QTw<BITS> *>(a.tw);
   const bool table = a.flags & 2;
   const int64_t ioff = !valid ? 0 : table ? static_cast<const int64_t *>(a.aux0)[member]
                                           : member * a.idist;
@@ -303,20 +308,17 @@ __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
                                           : member * a.odist;
   const P *in = static_cast<const P *>(a.in) + ioff;
   P *out = static_cast<P *>(a.out) + ooff;
-  const bool bwd = a.flags & 1;
   const int64_t is = a.istride, os = a.ostride;
-  using O = typename FixedEngine<BITS, TPT, PTS>::O;
+  using E = FixedEngine<BITS, TPT, PTS, BWD>;
+  using O = typename E::O;
   using V = typename O::V;
   // unit strides (the common layout) index without 64-bit multiplies
   auto go = [&](auto unit) {
     constexpr bool U = decltype(unit)::value;
-    FixedEngine<BITS, TPT, PTS>::run(
-        a.desc, lds, tab + 16, tab, t, valid,
-        [&](int e) {
-          const V v = O::in(in[U ? (int64_t)e : e * is]);
-          return bwd ? O::swap(v) : v;
-        },
-        [&](int e, V v) { out[U ? (int64_t)e : e * os] = O::out(bwd ? O::swap(v) : v); });
+    E::run(
+        a.desc, lds, tab, t, valid,
+        [&](int e) { return O::in(in[U ? (int64_t)e : e * is]); },
+        [&](int e, V v) { out[U ? (int64_t)e : e * os] = O::out(v); });
   };
   if (is == 1 && os == 1) go(std::true_type{});
   else go(std::false_type{});

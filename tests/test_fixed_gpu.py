@@ -4,8 +4,10 @@ oracle's restatement (oracle.c orc_fixed_c2c), through the C ABI.
 The oracle itself is pinned on the CPU (tests/test_oracle.py) by the
 reference's fixed-point checks: delta <-> constant known pairs and DFT/n
 within 6 log2(n) LSB (test/delta_constant_test.hpp:40-62,
-test/test_utils.hpp:132-151). Bit parity with the reference's generated
-NEON codelets is unpinned (their op order cannot be run or read back here).
+test/test_utils.hpp:132-151), and the generated device codelets are pinned
+there against the oracle's interpreter of the reference's op lists. What
+stays unpinned is bit parity with the reference's emitted NEON code after
+its peephole rewrites (print_algo.cpp:236-973), which cannot be run here.
 """
 import numpy as np
 import pytest

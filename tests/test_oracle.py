@@ -378,10 +378,22 @@ def test_fixed_tiny_exact(orc):
     # saturation: the constant 32767 over 16 points sums to 32768 -> 32767
     y = orc.fixed_c2c(np.full((16, 2), [32767, -32768], np.int32), -1, 15)
     assert y[0].tolist() == [32767, -32768] and np.abs(y[1:]).max() == 0
-    # backward = swap o forward o swap, exactly
+    # n = 8, delta at 1 (8000 >> 3 = 1000) through the Winograd z8 op list
+    # by hand: products by Q(cos pi/4) = 23170 round 23170000 / 2^15 to 707
+    # (the fused z8 add of two products rounds once, to 707 / -707)
+    x = np.zeros((8, 2), np.int32)
+    x[1] = (8000, 0)
+    y = orc.fixed_c2c(x, -1, 15)
+    assert y.tolist() == [[1000, 0], [707, -707], [0, -1000], [-707, -707],
+                          [-1000, 0], [-707, 707], [0, 1000], [707, 707]]
+    # a codelet-only length: backward stores forward output k at (n - k) mod n
+    # (print_algo.cpp:1486-1487), exactly
     rng = np.random.default_rng(5)
-    x = rng.integers(-32768, 32768, size=(256, 2)).astype(np.int32)
-    assert np.array_equal(orc.fixed_c2c(x, 1, 15), orc.fixed_c2c(x[:, ::-1], -1, 15)[:, ::-1])
+    for n in (2, 4, 8, 16):
+        x = rng.integers(-32768, 32768, size=(n, 2)).astype(np.int32)
+        f = orc.fixed_c2c(x, -1, 15)
+        b = orc.fixed_c2c(x, 1, 15)
+        assert np.array_equal(b, f[(-np.arange(n)) % n]), n
 
 
 def _fixed_checks_pass(orc, n, bits=15, seed=9):
@@ -406,3 +418,65 @@ def test_fixed_planted_mistakes_are_caught(orc):
     finally:
         orc.fixed_set_plant(0)
     assert _fixed_checks_pass(orc, 4096)
+
+
+@pytest.fixture(scope="module")
+def host_codelets(tmp_path_factory):
+    """The device codelets generated from the op lists
+    (perf-libs-fft_amd/csrc/fixed_codelets.hpp) built for the host with a
+    scalar ops policy (tests/c_abi/fixed_codelets_host.cpp)."""
+    import ctypes
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    so = tmp_path_factory.mktemp("fxc") / "libfxc.so"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-shared", "-fPIC", "-Wall", "-Werror",
+                    "-I", os.path.join(root, "perf-libs-fft_amd", "csrc"),
+                    os.path.join(root, "tests", "c_abi", "fixed_codelets_host.cpp"),
+                    "-o", str(so)], check=True, capture_output=True, text=True)
+    L = ctypes.CDLL(str(so))
+    ip = ctypes.POINTER(ctypes.c_int32)
+    L.host_fixed_codelet.argtypes = [ctypes.c_int, ctypes.c_int, ip, ip]
+    L.host_fixed_codelet.restype = ctypes.c_int
+
+    def run(x, bits):
+        a = np.ascontiguousarray(x, dtype=np.int32)
+        y = np.zeros_like(a)
+        assert L.host_fixed_codelet(a.shape[0], bits, a.ctypes.data_as(ip), y.ctypes.data_as(ip)) == 0
+        return y
+    return run
+
+
+@pytest.mark.parametrize("bits", [15, 7])
+@pytest.mark.parametrize("r", [2, 4, 8, 16])
+def test_generated_codelets_match_oracle_op_lists(orc, host_codelets, r, bits):
+    """The generated codelets (what the kernels run) and the oracle's
+    interpreter of the op lists agree bit for bit: on the shifted inputs a
+    codelet sees (|x| <= 2^(B - log2 R)), on the full storage range (where
+    saturation and the wrapping accumulator act), and on the extremes."""
+    rng_ = np.random.default_rng(100 * r + bits)
+    hi = 1 << bits
+    sh = r.bit_length() - 1
+    cases = [rng_.integers(-(hi >> sh), (hi >> sh) + 1, size=(r, 2)) for _ in range(300)]
+    cases += [rng_.integers(-hi, hi, size=(r, 2)) for _ in range(300)]
+    cases += [rng_.choice([-hi, hi - 1, 0, -1, 1], size=(r, 2)) for _ in range(200)]
+    for x in cases:
+        x = x.astype(np.int32)
+        assert np.array_equal(host_codelets(x, bits), orc.fixed_codelet(x, bits)), x.tolist()
+    # and the op lists compute the DFT: a delta and a unit tone
+    x = np.zeros((r, 2), np.int32)
+    x[0] = (hi >> sh, 0)
+    assert np.array_equal(orc.fixed_codelet(x, bits), np.tile([hi >> sh, 0], (r, 1)))
+
+
+def test_generated_codelet_known_value(orc, host_codelets):
+    """A value worked by hand: the radix-8 product by cos(pi/4), isolated by
+    a delta at input 1; one LSB less in the constant would change it."""
+    x = np.zeros((8, 2), np.int32)
+    x[1] = (32767, 0)
+    y = host_codelets(x, 15)
+    assert np.array_equal(y, orc.fixed_codelet(x, 15))
+    # Q(cos pi/4) = 23170: 32767 * 23170 / 2^15 = 23169.29 rounds to 23169;
+    # with 23169 it would be 23168
+    assert y[1].tolist() == [23169, -23169]
+    assert (32767 * 23169 + (1 << 14)) >> 15 == 23168

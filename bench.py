@@ -154,7 +154,7 @@ def progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def bench_mixed(torch, P, steps: int, warmup: int):
+def bench_mixed(torch, P, steps: int, warmup: int, fused_only: bool = False):
     """The mixed-length batch: 10,000 C2C members, N log-uniform in
     [16, 65536], 10 % moved to a prime, packed back to back, forward,
     out-of-place. Fused = one mixed plan, at most three launches (one per
@@ -212,6 +212,8 @@ def bench_mixed(torch, P, steps: int, warmup: int):
     res["rel_err"] = float(f"{math.sqrt(num / den):.3g}")
     res["rel_err_max_member"] = float(f"{worst:.3g}")
     res["rel_err_sample"] = "every 10th member vs numpy FP64 FFT"
+    if fused_only:  # counter collection: the f32 fused plan only
+        return res
 
     # 16-bit storage, FP32 arithmetic
     for dt in ("f16", "bf16"):
@@ -389,6 +391,8 @@ def main() -> int:
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true",
                     help="run only the mixed-length batch comparison")
+    ap.add_argument("--mixed-fused-only", action="store_true",
+                    help="run only the fused f32 mixed batch (counter collection)")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (counter calibration)")
@@ -409,9 +413,9 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
 
-    if args.mixed_only:
+    if args.mixed_only or args.mixed_fused_only:
         print(json.dumps({"mixed": bench_mixed(torch, P, max(3, args.steps // 10),
-                                               min(args.warmup, 3))}))
+                                               min(args.warmup, 3), args.mixed_fused_only)}))
         return 0
 
     total = 1 << 27  # complex elements per case

@@ -321,9 +321,11 @@ struct StaticEngineSplit {
 // ---------------------------------------------------------------------------
 // runtime radix chain (descriptor in device memory). BIGP enables the
 // radix 17/19/23 butterflies, which need more registers than a 1024-thread
-// block can give each thread.
+// block can give each thread. SMALL compiles only radices 2, 3, 4, 5, 7, 8
+// and 16 (7-smooth chains): a kernel that runs many engine instances keeps
+// its code small.
 // ---------------------------------------------------------------------------
-template <typename C, int TPT, int PTS, bool BIGP>
+template <typename C, int TPT, int PTS, bool BIGP, bool SMALL = false>
 struct DynamicEngine {
   static constexpr int PAD = 5;
 
@@ -423,18 +425,25 @@ struct DynamicEngine {
         PLF_DYN_CASE(5)
         PLF_DYN_CASE(7)
         PLF_DYN_CASE(8)
-        PLF_DYN_CASE(11)
-        PLF_DYN_CASE(13)
         PLF_DYN_CASE(16)
 #undef PLF_DYN_CASE
+        case 11:
+          if constexpr (!SMALL) stage<11, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
+          break;
+        case 13:
+          if constexpr (!SMALL) stage<13, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
+          break;
         case 17:
-          if constexpr (BIGP) stage<17, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
+          if constexpr (BIGP && !SMALL)
+            stage<17, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
           break;
         case 19:
-          if constexpr (BIGP) stage<19, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
+          if constexpr (BIGP && !SMALL)
+            stage<19, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
           break;
         case 23:
-          if constexpr (BIGP) stage<23, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
+          if constexpr (BIGP && !SMALL)
+            stage<23, TO_LDS>(d, s, lds, tw, t, valid, lds_in, load, store);
           break;
         default:
           break;

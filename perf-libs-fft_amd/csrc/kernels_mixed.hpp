@@ -26,9 +26,10 @@ struct MixedExec {
   using ST = StorageT<S>;
   using E = typename ST::E;
   using C = typename ST::C;
-  using EngD = DynamicEngine<C, 256, 16, true>;   // one member per block
-  using Eng16 = DynamicEngine<C, 16, 16, true>;   // 16 lanes x 16 threads
-  using Eng8 = DynamicEngine<C, 32, 16, true>;    // 8 lanes x 32 threads
+  // 7-smooth chains only (radices 2, 3, 4, 5, 7, 8, 16)
+  using EngD = DynamicEngine<C, 256, 16, true, true>;   // one member per block
+  using Eng16 = DynamicEngine<C, 16, 16, true, true>;   // 16 lanes x 16 threads
+  using Eng8 = DynamicEngine<C, 32, 16, true, true>;    // 8 lanes x 32 threads
   static constexpr int PAD = 5;
 
   // pass kinds of the tile functions
@@ -83,15 +84,16 @@ struct MixedExec {
         });
   }
 
-  // column pass over columns [first, first + LANES) of the n1 x n2 matrix
+  // column pass over columns [first, end) (at most LANES) of the n1 x n2
+  // matrix; the host starts tiles where the member's rows are line-aligned
   template <class EngT, int LANES>
   static __device__ __forceinline__ void col_tile(const MixedArgs &a, int pass, int first,
-                                                  const MixedXf &x, cpx<C> *lds_all) {
+                                                  int end, const MixedXf &x, cpx<C> *lds_all) {
     const int g = threadIdx.x % LANES;
     const int t = threadIdx.x / LANES;
     const int n1 = x.n1, n2 = x.n2;
     const int col = first + g;
-    const bool valid = col < n2;
+    const bool valid = col < end;
     cpx<C> *lds = lds_all + g * lds_padded<PAD>(n1);
     const GenericDesc *d = a.descs + x.d0;
     const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw) + a.desc_tw[x.d0];
@@ -177,7 +179,7 @@ struct MixedExec {
     if (valid)
       for (int k2 = t; k2 < n2; k2 += TPT) {
         const int p = lds_pad<PAD>(k2);
-        lds[p] = cswap(cmul(lds[p], B[row + (int64_t)n1 * k2]));
+        lds[p] = cswap(cmul(lds[p], B[(int64_t)row * n2 + k2]));  // B row-major by k1
       }
     __syncthreads();
     EngT::template run<false, true>(
@@ -185,6 +187,10 @@ struct MixedExec {
         [&](int m2, cpx<C> v) { rowp[m2] = cmul(v, split_tw(aux, x, (int64_t)row * m2)); });
   }
 
+  // The task types of one dependency level: 0 = ONE, COL, CZ_IN;
+  // 1 = ROW, CZ_MID; 2 = CZ_OUT. One kernel per level keeps each kernel's
+  // code to the engine instances that level runs.
+  template <int LEVEL>
   static __device__ __forceinline__ void run(const MixedArgs &a, cpx<C> *lds) {
     const int G = gridDim.x, b = blockIdx.x;
     for (int r = 0;; ++r) {
@@ -192,26 +198,25 @@ struct MixedExec {
       if (p >= a.ntasks) break;
       const MixedTask tk = a.tasks[p];
       const MixedXf x = a.xf[tk.xf];
-      switch (tk.type) {
-        case MT_ONE:
+      if constexpr (LEVEL == 0) {
+        if (tk.type == MT_ONE) {
           one_block(a, x, lds);
-          break;
-        case MT_COL:
-        case MT_CZ_IN:
-        case MT_CZ_OUT: {
-          const int pass = tk.type == MT_COL ? COL : (tk.type == MT_CZ_IN ? CZ_IN : CZ_OUT);
-          if (x.lanes0 == 16) col_tile<Eng16, 16>(a, pass, tk.first, x, lds);
-          else col_tile<Eng8, 8>(a, pass, tk.first, x, lds);
-          break;
+        } else {
+          const int pass = tk.type == MT_COL ? COL : CZ_IN;
+          if (x.lanes0 == 16) col_tile<Eng16, 16>(a, pass, tk.first, tk.end, x, lds);
+          else col_tile<Eng8, 8>(a, pass, tk.first, tk.end, x, lds);
         }
-        case MT_ROW:
+      } else if constexpr (LEVEL == 1) {
+        if (tk.type == MT_ROW) {
           if (x.lanes1 == 16) row_tile<Eng16, 16>(a, tk.first, x, lds);
           else row_tile<Eng8, 8>(a, tk.first, x, lds);
-          break;
-        default:  // MT_CZ_MID
+        } else {  // MT_CZ_MID
           if (x.lanes1 == 16) row_mid<Eng16, 16>(a, tk.first, x, lds);
           else row_mid<Eng8, 8>(a, tk.first, x, lds);
-          break;
+        }
+      } else {  // MT_CZ_OUT
+        if (x.lanes0 == 16) col_tile<Eng16, 16>(a, CZ_OUT, tk.first, tk.end, x, lds);
+        else col_tile<Eng8, 8>(a, CZ_OUT, tk.first, tk.end, x, lds);
       }
       __syncthreads();  // the next task rewrites the LDS this one read
     }
@@ -220,11 +225,12 @@ struct MixedExec {
 
 // At most 128 VGPRs (4 waves per SIMD): the blocks hide HBM latency by
 // occupancy, and LDS (about 34 KB per block for FP32) allows 4 per CU.
-template <typename S>
+template <typename S, int LEVEL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 k_mixed(MixedArgs a) {
   extern __shared__ unsigned char plf_dyn_lds[];
-  MixedExec<S>::run(a, reinterpret_cast<cpx<typename StorageT<S>::C> *>(plf_dyn_lds));
+  MixedExec<S>::template run<LEVEL>(
+      a, reinterpret_cast<cpx<typename StorageT<S>::C> *>(plf_dyn_lds));
 }
 
 }  // namespace plf

@@ -3,9 +3,12 @@
 
 namespace plf {
 
-hipError_t launch_mixed_f32(const MixedArgs &a, int grid, size_t lds, hipStream_t s) {
-  return launch_mixed_t<float>(a, grid, lds, s);
+hipError_t launch_mixed_f32(int level, const MixedArgs &a, int grid, size_t lds,
+                            hipStream_t s) {
+  return launch_mixed_t<float>(level, a, grid, lds, s);
 }
-int mixed_blocks_per_cu_f32(size_t lds) { return mixed_blocks_per_cu_t<float>(lds); }
+int mixed_blocks_per_cu_f32(int level, size_t lds) {
+  return mixed_blocks_per_cu_t<float>(level, lds);
+}
 
 }  // namespace plf

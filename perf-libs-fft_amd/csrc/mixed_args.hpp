@@ -21,8 +21,8 @@ struct MixedXf {
   int64_t in_off, out_off;  // complex elements
   int64_t ws_off;           // complex elements of workspace (kind 1: N; kind 3: M)
   int64_t tw4_off;          // split twiddles in aux: lo (2^shift), then hi
-  int64_t chirp_off;        // chirp c_k (n entries) in aux
-  int64_t B_off;            // B = DFT_M(b)/M (M entries) in aux
+  int64_t chirp_off;        // chirp c_k (n entries) in aux, at the residue of in_off mod 16
+  int64_t B_off;            // B = DFT_M(b)/M (M entries) in aux; chirp-z: row-major n1 x n2
 };
 
 enum MixedTaskType : int {
@@ -38,6 +38,7 @@ struct MixedTask {
   int type;
   int xf;     // member
   int first;  // first column / row of the tile
+  int end;    // column tiles: one past the last column
 };
 
 struct MixedArgs {
@@ -54,9 +55,11 @@ struct MixedArgs {
   int bwd;                 // 1: backward transform (exponent sign +)
 };
 
-// Launch the mixed-batch kernel for storage precision `prec` (Prec values).
-hipError_t launch_mixed(int prec, const MixedArgs &a, int grid, size_t lds, hipStream_t s);
-// Resident blocks per CU of the mixed-batch kernel with `lds` bytes of LDS.
-int mixed_blocks_per_cu(int prec, size_t lds);
+// Launch the mixed-batch kernel of dependency level `level` (0, 1, 2) for
+// storage precision `prec` (Prec values).
+hipError_t launch_mixed(int prec, int level, const MixedArgs &a, int grid, size_t lds,
+                        hipStream_t s);
+// Resident blocks per CU of that kernel with `lds` bytes of LDS.
+int mixed_blocks_per_cu(int prec, int level, size_t lds);
 
 }  // namespace plf

@@ -31,24 +31,26 @@ constexpr int64_t kMaxSide = 512;    // longest side of a four-step tile pass
 
 int64_t padded(int64_t n) { return n + (n >> 5) + 1; }
 
+// The mixed kernels compile radices 2, 3, 4, 5, 7, 8 and 16 only (fewer
+// engine instances keep each level's kernel small), so every engine length
+// is 7-smooth; other lengths take Bluestein.
 bool radix_ok(int r) {
   switch (r) {
-    case 2: case 3: case 4: case 5: case 7: case 8: case 11: case 13: case 16:
-    case 17: case 19: case 23:
+    case 2: case 3: case 4: case 5: case 7: case 8: case 16:
       return true;
     default:
       return false;
   }
 }
 
-std::vector<int> chain_of(int64_t n) { return n == 1 ? std::vector<int>{} : dynamic_chain(n); }
-bool smooth(int64_t n) { return n == 1 || !dynamic_chain(n).empty(); }
-
 bool smooth7(int64_t v) {
   for (int p : {2, 3, 5, 7})
     while (v % p == 0) v /= p;
   return v == 1;
 }
+
+std::vector<int> chain_of(int64_t n) { return n == 1 ? std::vector<int>{} : dynamic_chain(n); }
+bool smooth(int64_t n) { return n == 1 || smooth7(n); }
 
 // Every stage of the chain fits `tpt` threads with kPts points each.
 bool engine_fits(int64_t n, int tpt) {
@@ -78,10 +80,14 @@ struct Split {
 
 // Predicted cost of the two tile passes of a split, in element units: each
 // tile (one block task) costs a fixed overhead plus the elements it moves.
+// A column tile reads LANES consecutive elements of each row; when n2 is not
+// a multiple of LANES the rows fall on different line offsets and a tile's
+// row segments straddle two lines, which costs about twice the reads.
 constexpr int64_t kTileOverhead = 4096;
 int64_t split_cost(const Split &sp) {
-  const int64_t ncol = (sp.n2 + sp.l0 - 1) / sp.l0, nrow = (sp.n1 + sp.l1 - 1) / sp.l1;
-  return ncol * (kTileOverhead + sp.l0 * sp.n1) + nrow * (kTileOverhead + sp.l1 * sp.n2);
+  const int64_t ncol = (sp.n2 + sp.l0 - 1) / sp.l0 + 1, nrow = (sp.n1 + sp.l1 - 1) / sp.l1;
+  const int64_t colmul = (sp.n2 % sp.l0) ? 2 : 1;
+  return ncol * (kTileOverhead + colmul * sp.l0 * sp.n1) + nrow * (kTileOverhead + sp.l1 * sp.n2);
 }
 
 // The split N = n1 * n2 with the least predicted tile cost, then the most
@@ -115,7 +121,7 @@ struct Schedule {
 };
 
 bool schedule_for(int64_t n, Schedule *s, std::string *why) {
-  if (n <= kOneBlock && engine_fits(n, 256)) {
+  if (n <= kOneBlock && smooth(n) && engine_fits(n, 256)) {
     s->kind = 0;
     s->len = n;
     return true;
@@ -232,7 +238,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
   // aux: split twiddles per four-step length, chirps and B per Bluestein n
   std::vector<double> aux;
   std::map<int64_t, std::pair<int64_t, int>> split_of;  // N -> (offset, shift)
-  std::map<int64_t, int64_t> chirp_of, B_of;
+  std::map<int64_t, int64_t> B_of;
   auto get_split = [&](int64_t N) -> std::pair<int64_t, int> {
     auto it = split_of.find(N);
     if (it != split_of.end()) return it->second;
@@ -255,16 +261,22 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     }
     return split_of[N] = {off, shift};
   };
-  // Bluestein members grouped by M, for batched FP64 DFTs of b
+  // Bluestein members grouped by M, for batched FP64 DFTs of b. Chirp
+  // tables are placed per member below (aligned like the member's data).
+  std::map<int64_t, std::vector<double>> chirp_host;
   std::map<int64_t, std::vector<int64_t>> blue_by_M;
   for (const auto &kv : sched)
     if (kv.second.kind >= 2) {
-      const int64_t nn = kv.first;
-      chirp_of[nn] = (int64_t)(aux.size() / 2);
-      const std::vector<double> c = chirp_table(nn);
-      aux.insert(aux.end(), c.begin(), c.end());
-      blue_by_M[kv.second.len].push_back(nn);
+      chirp_host[kv.first] = chirp_table(kv.first);
+      blue_by_M[kv.second.len].push_back(kv.first);
     }
+  // pad aux (complex entries) up to an offset congruent to r mod 16
+  auto align_aux = [&](int64_t r) {
+    while ((int64_t)(aux.size() / 2) % 16 != r) {
+      aux.push_back(0.0);
+      aux.push_back(0.0);
+    }
+  };
   for (const auto &kv : blue_by_M) {
     const int64_t M = kv.first;
     const std::vector<int64_t> &ns = kv.second;
@@ -272,7 +284,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     const double s = 1.0 / (double)M;
     for (size_t j = 0; j < ns.size(); ++j) {
       const int64_t nn = ns[j];
-      const double *c = aux.data() + 2 * chirp_of[nn];
+      const double *c = chirp_host[nn].data();
       double *row = b.data() + 2 * (size_t)M * j;
       for (int64_t k = 0; k < nn; ++k) {
         const double re = c[2 * k] * s, im = -c[2 * k + 1] * s;  // conj(c_k) / M
@@ -290,10 +302,35 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
       return nullptr;
     }
     for (size_t j = 0; j < ns.size(); ++j) {
+      const Schedule &sj = sched[ns[j]];
+      const double *src = b.data() + 2 * (size_t)M * j;
+      align_aux(0);
       B_of[ns[j]] = (int64_t)(aux.size() / 2);
-      aux.insert(aux.end(), b.begin() + 2 * (size_t)M * j, b.begin() + 2 * (size_t)M * (j + 1));
+      if (sj.kind == 3) {
+        // chirp-z: row k1 of the middle pass reads B[k1 + n1 k2], k2 < n2,
+        // so B is stored row-major by k1
+        const int64_t n1 = sj.sp.n1, n2 = sj.sp.n2;
+        for (int64_t k1 = 0; k1 < n1; ++k1)
+          for (int64_t k2 = 0; k2 < n2; ++k2) {
+            aux.push_back(src[2 * (k1 + n1 * k2)]);
+            aux.push_back(src[2 * (k1 + n1 * k2) + 1]);
+          }
+      } else {
+        aux.insert(aux.end(), src, src + 2 * (size_t)M);
+      }
     }
   }
+  // chirp tables per (length, residue of the member's offset mod 16)
+  std::map<std::pair<int64_t, int64_t>, int64_t> chirp_at;
+  auto get_chirp = [&](int64_t nn, int64_t r) -> int64_t {
+    auto it = chirp_at.find({nn, r});
+    if (it != chirp_at.end()) return it->second;
+    align_aux(r);
+    const int64_t off = (int64_t)(aux.size() / 2);
+    const std::vector<double> &c = chirp_host[nn];
+    aux.insert(aux.end(), c.begin(), c.end());
+    return chirp_at[{nn, r}] = off;
+  };
 
   // members, their workspace, and their tasks per dependency level with a
   // predicted cost (HBM elements moved plus an LDS-work term)
@@ -321,7 +358,10 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     P.out_span_ = std::max(P.out_span_, x.out_off + nn);
     x.d0 = x.d1 = 0;
     x.ws_off = 0;
-    x.chirp_off = chirp_of.count(nn) ? chirp_of[nn] : 0;
+    // the member's line offset: workspace rows, chirp tables and column
+    // tiles are placed at the same residue mod 16 as its input
+    const int64_t r16 = ((x.in_off % 16) + 16) % 16;
+    x.chirp_off = s.kind >= 2 ? get_chirp(nn, r16) : 0;
     x.B_off = B_of.count(nn) ? B_of[nn] : 0;
     ++nkind[s.kind];
     const int xi = (int)i;
@@ -329,7 +369,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
       x.d0 = get_desc(s.len);
       lds = std::max(lds, (size_t)padded(s.len) * csize);
       const double lwork = 0.1 * (double)s.len * std::log2((double)std::max<int64_t>(2, s.len));
-      level[0].push_back({MixedTask{MT_ONE, xi, 0}, 2.0 * (double)nn + lwork});
+      level[0].push_back({MixedTask{MT_ONE, xi, 0, 0}, 2.0 * (double)nn + lwork});
       continue;
     }
     const Split &sp = s.sp;
@@ -342,6 +382,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     const auto spl = get_split(s.len);
     x.tw4_off = spl.first;
     x.shift = spl.second;
+    ws += ((r16 - ws % 16) + 16) % 16;
     x.ws_off = ws;
     ws += s.len;
     lds = std::max(lds, (size_t)sp.l0 * padded(sp.n1) * csize);
@@ -349,12 +390,18 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     const double ccost = sp.l0 * (double)sp.n1 * (2.0 + 0.1 * std::log2((double)sp.n1));
     const double rcost = sp.l1 * (double)sp.n2 * (2.0 + 0.1 * std::log2((double)sp.n2));
     const bool cz = s.kind == 3;
-    for (int64_t c = 0; c < sp.n2; c += sp.l0) {
-      level[0].push_back({MixedTask{cz ? MT_CZ_IN : MT_COL, xi, (int)c}, ccost});
-      if (cz) level[2].push_back({MixedTask{MT_CZ_OUT, xi, (int)c}, ccost});
+    // column tiles start where in_off + column is a multiple of the tile
+    // width (a short head tile first)
+    const int64_t head = (sp.l0 - r16 % sp.l0) % sp.l0;
+    for (int64_t c = 0; c < sp.n2;) {
+      const int64_t e = std::min<int64_t>(sp.n2, c < head ? head : c + sp.l0);
+      level[0].push_back({MixedTask{cz ? MT_CZ_IN : MT_COL, xi, (int)c, (int)e}, ccost});
+      if (cz) level[2].push_back({MixedTask{MT_CZ_OUT, xi, (int)c, (int)e}, ccost});
+      c = e;
     }
     for (int64_t r = 0; r < sp.n1; r += sp.l1)
-      level[1].push_back({MixedTask{cz ? MT_CZ_MID : MT_ROW, xi, (int)r}, cz ? 2 * rcost : rcost});
+      level[1].push_back(
+          {MixedTask{cz ? MT_CZ_MID : MT_ROW, xi, (int)r, (int)sp.n1}, cz ? 2 * rcost : rcost});
   }
   std::vector<MixedTask> tasks;
   for (int L = 0; L < 3; ++L) {
@@ -382,12 +429,15 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus <= 0)
     cus = 256;
-  const int per_cu = mixed_blocks_per_cu(prec, P.lds_);
-  if (per_cu <= 0) {
-    *why = "mixed batch: the kernel does not fit a CU with this LDS size";
-    return nullptr;
+  for (int L = 0; L < 3; ++L) {
+    const int per_cu = mixed_blocks_per_cu(prec, L, P.lds_);
+    if (per_cu <= 0) {
+      *why = "mixed batch: the kernel does not fit a CU with this LDS size";
+      return nullptr;
+    }
+    P.level_grid_[L] = per_cu * cus;
+    P.grid_ = std::max(P.grid_, P.level_grid_[L]);
   }
-  P.grid_ = per_cu * cus;
 
   std::ostringstream o;
   o << "(mixed count=" << count << " direct=" << nkind[0] << " fourstep=" << nkind[1]
@@ -421,8 +471,8 @@ hipError_t MixedPlan::enqueue(const void *in, void *out, void *ws, hipStream_t s
     if (level_count_[L] == 0) continue;
     a.tasks = static_cast<const MixedTask *>(tasks_.p) + level_first_[L];
     a.ntasks = level_count_[L];
-    const int grid = (int)std::min<int64_t>(level_count_[L], grid_);
-    const hipError_t err = launch_mixed(prec_, a, grid, lds_, s);
+    const int grid = (int)std::min<int64_t>(level_count_[L], level_grid_[L]);
+    const hipError_t err = launch_mixed(prec_, L, a, grid, lds_, s);
     if (err != hipSuccess) return err;
   }
   return hipSuccess;

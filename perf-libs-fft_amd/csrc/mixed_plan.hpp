@@ -41,6 +41,7 @@ class MixedPlan {
   int64_t count_ = 0;
   int64_t in_span_ = 0, out_span_ = 0;
   int grid_ = 0;  // resident blocks (largest grid of a launch)
+  int level_grid_[3] = {0, 0, 0};  // resident blocks of each level's kernel
   int64_t level_first_[3] = {0, 0, 0}, level_count_[3] = {0, 0, 0};
   size_t lds_ = 0, data_bytes_ = 0;
   DevMem xf_, tasks_, descs_, desc_tw_, tw_, aux_;

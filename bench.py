@@ -297,6 +297,7 @@ def case_specs(total: int):
         return sp
 
     specs = [
+        c("c2c", 16), c("c2c", 32), c("c2c", 16, "f16"),
         c("c2c", 256), c("c2c", 1024, batch=4096), c("c2c", 4096, d=1),
         c("c2c", 4096, inplace=True), c("c2c", 16384), c("c2c", 65536),
         c("c2c", 65536, d=1, inplace=True), c("c2c", 1 << 20),

@@ -1,4 +1,5 @@
 // kernels_bf16.hip -- bf16 storage with FP32 arithmetic.
+#include "kernels_xlane.hpp"
 #include "launch.hpp"
 
 namespace plf {
@@ -31,6 +32,8 @@ void register_kernels_bf16(std::vector<KEntry> &r) {
   PLF_DYNAMIC(k_row, KK_ROW, S, P,  256,   4, 16);
   // unit stride: non-temporal loads and stores, no stride arithmetic
   //              TPT  TPB PAD PAIR   radix chain
+  PLF_XLANE(S, P, 16, 2, 2, 2, 2);  // cross-lane, packed members
+  PLF_XLANE(S, P, 32, 2, 2, 2, 2, 2);
   PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
   PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
   // 4096 and 16384: the split (real, then imaginary) LDS exchange first

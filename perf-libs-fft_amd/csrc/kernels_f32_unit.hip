@@ -1,5 +1,6 @@
 // kernels_f32_unit.hip -- FP32 unit-stride streaming C2C configurations
 // (first entry per length is the default; the autotuner may pick another).
+#include "kernels_xlane.hpp"
 #include "launch.hpp"
 
 namespace plf {
@@ -7,6 +8,12 @@ namespace plf {
 void register_kernels_f32_unit(std::vector<KEntry> &r) {
   using S = float;
   constexpr int P = P_F32;
+  // n = 16, 32 with packed members: butterflies across lanes (DPP,
+  // ds_swizzle), no LDS exchange. At 2^27 points: n = 16 0.35 ms (LDS
+  // kernel 0.84), n = 32 0.36 ms (0.47); n = 64 measured 0.41 ms against
+  // 0.39 for the LDS radix-8x8 kernel, so it is not registered.
+  PLF_XLANE(S, P, 16, 2, 2, 2, 2);
+  PLF_XLANE(S, P, 32, 2, 2, 2, 2, 2);
   //              TPT  TPB PAD PAIR   radix chain
   PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
   PLF_UNIT(S, P,   8,  32, 4, true,  16, 16);

@@ -154,6 +154,18 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
     r[r.size() - 2].pad = PAD;                                                            \
   } while (0)
 
+// Cross-lane C2C of n = 16, 32, 64 (k_c2c_xlane): 256-thread blocks of four
+// waves, each wave 1024 / n packed members; the chain names the radix-2
+// stages.
+#define PLF_XLANE(S, PREC, N, ...)                                                         \
+  do {                                                                                     \
+    ::plf::add_entry(r, ::plf::KK_C2C_UNIT, PREC, N, {__VA_ARGS__}, N / 16, 4096 / N, 0,  \
+                     false, &::plf::launch_k<&::plf::k_c2c_xlane<S, false, N>, 256>,       \
+                     &::plf::launch_k<&::plf::k_c2c_xlane<S, true, N>, 256>);             \
+    r[r.size() - 1].xlane = true;                                                          \
+    r[r.size() - 2].xlane = true;                                                          \
+  } while (0)
+
 // Unit-stride software-pipelined C2C (k_c2c_pipe), PER_CU blocks per CU.
 #define PLF_PIPE(S, PREC, TPT, PAD, PER_CU, ...)                                           \
   do {                                                                                    \

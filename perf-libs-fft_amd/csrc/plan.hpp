@@ -100,7 +100,10 @@ struct Choice {
 };
 // Compile-time chain first (registry order = preference), else the dynamic
 // entry with the fewest threads that fits; e == null if none.
-Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div = 0);
+// packed: members contiguous at distance n with unit stride (cross-lane
+// kernels need it)
+Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div = 0,
+              bool packed = false);
 std::vector<int> dynamic_chain(int64_t n);
 std::string chain_str(const Choice &c);
 // Upload a FP64 table (interleaved re/im) in the compute precision.
@@ -144,6 +147,10 @@ int64_t fourstep_chunk(int64_t n, int prec, int64_t batch);
 // for stage s >= 1 with radix R and L = product of earlier radices,
 // tw[off_s + (m-1)*L + j] = exp(-2*pi*i*j*m/(L*R)), m in [1,R), j in [0,L).
 std::vector<double> stockham_twiddles(const std::vector<int> &chain);
+// Lane twiddles of the cross-lane kernels (n = 16, 32, 64): row r for the
+// decimation-in-frequency stage of half-size h = n / 2^(r+1) (h >= 2), entry
+// j = W_{2h}^{j mod h} where bit h of j is set, else 1.
+std::vector<double> xlane_twiddles(int64_t n);
 
 // Bluestein chirp c_k = exp(-i pi (k^2 mod 2n)/n), k < n, interleaved FP64.
 std::vector<double> chirp_table(int64_t n);

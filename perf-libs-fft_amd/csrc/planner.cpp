@@ -56,6 +56,7 @@ std::string KEntry::name() const {
   if (lds_scalars == 1) o << "_split";
   if (persist) o << "_pipe" << persist;
   if (waves) o << "_w" << waves;
+  if (xlane) o << "_xlane";
   return o.str();
 }
 
@@ -117,11 +118,12 @@ std::vector<int> dynamic_chain(int64_t n) {
 
 // Compile-time chain first (registry order = preference), else the dynamic
 // entry with the fewest threads that fits.
-Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div) {
+Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div, bool packed) {
   Choice c;
   std::vector<Choice> stat;
   for (const auto &e : kernel_registry()) {
     if (e.kind != kind || e.prec != prec || e.bwd != bwd || e.n != len) continue;
+    if (e.xlane && !packed) continue;
     if (need_tpb_div && need_tpb_div % e.tpb) continue;
     if (e.lds_bytes(len) > 160 * 1024) continue;
     Choice s;
@@ -157,6 +159,7 @@ hipError_t upload_table(DevMem &dst, const std::vector<double> &v, int prec) {
 }
 
 hipError_t EngineData::init(const Choice &c, int64_t len) {
+    if (c.e->xlane) return upload_table(tw, xlane_twiddles(len), c.e->prec);
     hipError_t err = upload_table(tw, stockham_twiddles(c.chain), c.e->prec);
     if (err != hipSuccess || c.e->n) return err;
     GenericDesc d{};
@@ -353,7 +356,9 @@ std::unique_ptr<Node> build_c2c_node(const Problem &p, std::string *why) {
   const int64_t od = p.howmany > 1 ? p.odist : 0;
   // 1. one fused kernel (the streaming unit-stride variant when it applies)
   Choice c;
-  if (p.istride == 1 && p.ostride == 1) c = choose(KK_C2C_UNIT, p.n, p.prec, bwd);
+  const bool packed = p.istride == 1 && p.ostride == 1 &&
+                      (p.howmany <= 1 || (p.idist == p.n && p.odist == p.n));
+  if (p.istride == 1 && p.ostride == 1) c = choose(KK_C2C_UNIT, p.n, p.prec, bwd, 0, packed);
   if (!c.e) c = choose(KK_C2C, p.n, p.prec, bwd);
   if (c.e) {
     auto node = std::make_unique<BatchC2CNode>(c, p.n, p.howmany, p.istride, id, p.ostride, od);

@@ -64,6 +64,7 @@ struct KEntry {
   int lds_scalars = 2;  // LDS scalars per element: 2, or 1 for the split exchange
   int persist = 0;      // > 0: persistent launch with at most this many blocks per CU
   int waves = 0;        // > 0: compiled for at least this many waves per SIMD
+  bool xlane = false;   // cross-lane kernel: packed members only, lane twiddle table
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len

@@ -66,3 +66,22 @@ void set_last_error(const std::string &s) { g_last_error = s; }
 const char *last_error() { return g_last_error.c_str(); }
 
 }  // namespace plf
+
+namespace plf {
+
+std::vector<double> xlane_twiddles(int64_t n) {
+  std::vector<double> tw;
+  for (int64_t h = n / 2; h >= 2; h /= 2)
+    for (int64_t j = 0; j < n; ++j) {
+      double re = 1.0, im = 0.0;
+      if (j & h) {
+        unit_root(j % h, 2 * h, &re, &im);  // exp(+2 pi i p / 2h)
+        im = -im;
+      }
+      tw.push_back(re);
+      tw.push_back(im);
+    }
+  return tw;
+}
+
+}  // namespace plf

@@ -19,6 +19,11 @@ pytestmark = pytest.mark.gpu
 
 # (n, name substring of the variant)
 VARIANTS = [
+    (16, "n16_r2x2x2x2_t1_b256_p5_xlane"),
+    (32, "n32_r2x2x2x2x2_t2_b128_p5_xlane"),
+    (16, "n16_r16_t1_b256"),
+    (32, "n32_r8x4_t4_b64"),
+    (64, "n64_r16x4_t4_b64"),
     (256, "n256_r16x16_t16_b16_p4"),
     (256, "n256_r16x16_t8_b32_p4_pair"),
     (1024, "n1024_r16x16x4_t64_b4_p4"),
@@ -58,3 +63,28 @@ def test_forced_variant_vs_oracle(torch_cuda, orc, monkeypatch, n, name, directi
     y = from_dev(torch, yd)
     ref = orc.c2c_batched(x.astype(np.complex128), n, direction, hm, 1, n, 1, n, n * hm)
     assert relerr(y, ref) < TOL["f32"], plan.describe()
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_cross_lane_kernel_layouts(torch_cuda, orc, n):
+    """The cross-lane kernels take packed members only: a batch at distance
+    n (and in place) runs them, a wider distance falls back to the LDS
+    kernels; all match the oracle."""
+    torch = torch_cuda
+    hm = 4099  # a partial last wave
+    x = rng.complex_normal(7 * n + hm, (n + 3) * hm).astype(np.complex64)
+    for dist in (n, n + 3):
+        plan = P.Plan("c2c", n, -1, "f32", howmany=hm, idist=dist, odist=dist)
+        assert ("xlane" in plan.describe()) == (dist == n), plan.describe()
+        xd = to_dev(torch, x[:dist * hm])
+        yd = torch.zeros_like(xd)
+        plan.execute(xd, yd)
+        y = from_dev(torch, yd)
+        ref = orc.c2c_batched(x[:dist * hm].astype(np.complex128), n, -1, hm, 1, dist, 1, dist,
+                              dist * hm)
+        assert relerr(y, ref) < TOL["f32"], plan.describe()
+    plan = P.Plan("c2c", n, 1, "f32", howmany=hm, may_alias=True)
+    xd = to_dev(torch, x[:n * hm])
+    plan.execute(xd, xd)
+    ref = orc.c2c_batched(x[:n * hm].astype(np.complex128), n, 1, hm, 1, n, 1, n, n * hm)
+    assert relerr(from_dev(torch, xd), ref) < TOL["f32"]

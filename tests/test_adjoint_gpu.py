@@ -125,3 +125,59 @@ def test_differentiable_r2c_loss(torch_cuda, orc):
         fd = (_loss_fp64(xp, w, n, hm, is_, id_) - _loss_fp64(xm, w, n, hm, is_, id_)) / (2 * h)
         assert abs(fd - ref[j]) <= 1e-4 * max(1.0, abs(ref[j]))
         assert abs(g[j] - ref[j]) <= 2e-3 * max(1.0, abs(ref[j]))
+
+
+def _r2r_matrix(kind, n):
+    """FFTW's REDFT10 (DCT-II) / RODFT10 (DST-II) as explicit FP64 matrices:
+    Y_k = 2 sum_j x_j cos(pi (2j+1) k / 2n), Y_k = 2 sum_j x_j sin(pi (2j+1) (k+1) / 2n)."""
+    j = np.arange(n)[None, :]
+    k = np.arange(n)[:, None]
+    if kind == "dct2":
+        return 2.0 * np.cos(np.pi * (2 * j + 1) * k / (2.0 * n))
+    return 2.0 * np.sin(np.pi * (2 * j + 1) * (k + 1) / (2.0 * n))
+
+
+@pytest.mark.parametrize("r2r", ["dct2", "dst2"])
+def test_differentiable_r2r_loss(torch_cuda, r2r):
+    """loss = sum w_k Y_k^2, Y = DCT-II / DST-II of x at the differentiable
+    layout (n = 4096, input stride 2, distance 2n+64, batch reduced); f32
+    autograd against the closed form 2 A^T (w Y) from the explicit FP64
+    matrix A, and the closed form against FP64 central differences."""
+    torch = torch_cuda
+    from plfft_amd.autograd import PlanFunction
+    n, hm, is_, id_ = 4096, 4, 2, 2 * 4096 + 64
+    plan = P.Plan("r2r", n, -1, "f32", hm, is_, id_, 1, n, r2r=r2r)
+    isz = span(n, is_, hm, id_)
+    x = rng.normal_f64(6, isz)
+    w = rng.uniform_f64(7, hm * n).reshape(hm, n) * 0.5 + 0.5
+    xt = torch.tensor(x.astype(np.float32), device="cuda", requires_grad=True)
+    wt = torch.tensor(w.ravel().astype(np.float32), device="cuda")
+    Y = PlanFunction.apply(xt, plan)
+    loss = (wt * Y ** 2).sum()
+    loss.backward()
+    g = xt.grad.cpu().numpy().astype(np.float64)
+    A = _r2r_matrix(r2r, n)
+    ref = np.zeros(isz)
+    rows = []
+    for b in range(hm):
+        idx = b * id_ + is_ * np.arange(n)
+        Yb = A @ x[idx]
+        rows.append(Yb)
+        ref[idx] = 2.0 * (A.T @ (w[b] * Yb))
+    # the forward values too, against the same matrix
+    Yg = Y.detach().cpu().numpy().astype(np.float64).reshape(hm, n)
+    assert relerr(Yg, np.stack(rows)) < 2e-6
+    assert relerr(g, ref) < 2e-5
+
+    def loss64(xx):
+        return sum(float(np.sum(w[b] * (A @ xx[b * id_ + is_ * np.arange(n)]) ** 2))
+                   for b in range(hm))
+
+    for jj in (0, 1, 4095 * 2, 2 * id_ + 2 * 33):
+        h = 1e-4
+        xp, xm = x.copy(), x.copy()
+        xp[jj] += h
+        xm[jj] -= h
+        fd = (loss64(xp) - loss64(xm)) / (2 * h)
+        assert abs(fd - ref[jj]) <= 1e-4 * max(1.0, abs(ref[jj]))
+        assert abs(g[jj] - ref[jj]) <= 2e-3 * max(1.0, abs(ref[jj]))

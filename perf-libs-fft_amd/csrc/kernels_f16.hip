@@ -1,4 +1,5 @@
 // kernels_f16.hip -- f16 storage with FP32 arithmetic.
+#include "kernels_mfma.hpp"
 #include "kernels_xlane.hpp"
 #include "launch.hpp"
 
@@ -42,6 +43,18 @@ void register_kernels_f16(std::vector<KEntry> &r) {
   // the HBM peak against 0.29 with the plain exchange); 16384 at 64
   // registers (8 waves per SIMD, two blocks per CU)
   PLF_UNIT_SPLIT(S, P, 256, 1, 5, 16, 16, 16);
+  // the same chain with stage 0 as a matrix-core product (f16 operands,
+  // f32 accumulation): a candidate of the timed planning race, not the
+  // default. At 2^27 points it takes 0.716 ms against 0.358 for the plain
+  // kernel: its 70 VGPRs and 32 AGPRs leave 4 waves per SIMD instead of 7,
+  // and the kernel is latency-bound, so the saved butterflies do not pay.
+  ::plf::add_entry(r, KK_C2C_UNIT, P, 4096, {16, 16, 16}, 256, 1, 0, true,
+                   &launch_k<&k_c2c_mfma16<false>, 256>, &launch_k<&k_c2c_mfma16<true>, 256>);
+  for (int i = 1; i <= 2; ++i) {
+    r[r.size() - i].pad = 5;
+    r[r.size() - i].lds_scalars = 1;
+    r[r.size() - i].mfma = true;
+  }
   PLF_UNIT_SPLIT_OCC(S, P, 1024, 1, 5, 8, 16, 16, 16, 4);
   PLF_UNIT_SPLIT(S, P, 1024, 1, 5, 16, 16, 16, 4);
   PLF_UNIT(S, P, 256,   1, 4, false, 16, 16, 16);

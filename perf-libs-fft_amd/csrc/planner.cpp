@@ -57,6 +57,7 @@ std::string KEntry::name() const {
   if (persist) o << "_pipe" << persist;
   if (waves) o << "_w" << waves;
   if (xlane) o << "_xlane";
+  if (mfma) o << "_mfma";
   return o.str();
 }
 

@@ -65,6 +65,7 @@ struct KEntry {
   int persist = 0;      // > 0: persistent launch with at most this many blocks per CU
   int waves = 0;        // > 0: compiled for at least this many waves per SIMD
   bool xlane = false;   // cross-lane kernel: packed members only, lane twiddle table
+  bool mfma = false;    // first stage on the matrix cores
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len

@@ -84,3 +84,25 @@ def test_r2r_half(torch_cuda, orc, prec, kind):
     y = _decode(_run(torch, plan, xs, n * hm), prec)
     ref = orc.r2r_batched(k, xv, n, hm, 1, n, 1, n, n * hm)
     assert relerr(y, ref) < HTOL[prec], plan.describe()
+
+
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_c2c_f16_matrix_core_stage(torch_cuda, orc, monkeypatch, direction):
+    """The 4096-point f16 kernel whose first radix-16 stage runs on the
+    matrix cores (f16 operands, DFT matrix split into f16 high and low
+    parts, f32 accumulation), forced: it matches the FP64 oracle to the f16
+    tolerance and the plain f32-butterfly kernel to within output rounding."""
+    torch = torch_cuda
+    n, hm = 4096, 37
+    xs, xv = _encode(rng.normal_f64(11, 2 * n * hm), "f16")
+    ref = orc.c2c_batched(xv[0::2] + 1j * xv[1::2], n, direction, hm, 1, n, 1, n, n * hm)
+    outs = {}
+    for name in ("_mfma", "n4096_r16x16x16_t256_b1_p5_split"):
+        monkeypatch.setenv("PLFFT_KERNEL_FILTER", name)
+        plan = P.Plan("c2c", n, direction, "f16", howmany=hm)
+        assert name in plan.describe(), plan.describe()
+        y = _decode(_run(torch, plan, xs, 2 * n * hm), "f16")
+        outs[name] = y[0::2] + 1j * y[1::2]
+        assert relerr(outs[name], ref) < HTOL["f16"], plan.describe()
+    # the two kernels differ only in f32 rounding before the f16 store
+    assert relerr(outs["_mfma"], outs["n4096_r16x16x16_t256_b1_p5_split"]) < 3e-4

@@ -34,6 +34,18 @@ void register_kernels_f16(std::vector<KEntry> &r) {
   // unit stride: non-temporal loads and stores, no stride arithmetic
   //              TPT  TPB PAD PAIR   radix chain
   PLF_XLANE(S, P, 16, 2, 2, 2, 2);  // cross-lane, packed members
+  // n = 16 as one matrix-core product per 32 packed members: a candidate of
+  // the timed planning race behind the cross-lane kernel (2^27 points:
+  // 0.338 ms against 0.272; both are bound by loads in flight, not by the
+  // butterflies the matrix cores take over)
+  ::plf::add_entry(r, KK_C2C_UNIT, P, 16, {16}, 1, 256, 0, false,
+                   &launch_pipe<&k_c2c_mfma16_small<false>, 256, 6>,
+                   &launch_pipe<&k_c2c_mfma16_small<true>, 256, 6>);
+  for (int i = 1; i <= 2; ++i) {
+    r[r.size() - i].mfma = true;
+    r[r.size() - i].packed = true;
+    r[r.size() - i].persist = 6;
+  }
   PLF_XLANE(S, P, 32, 2, 2, 2, 2, 2);
   PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
   PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);

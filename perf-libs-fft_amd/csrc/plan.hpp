@@ -151,6 +151,11 @@ std::vector<double> stockham_twiddles(const std::vector<int> &chain);
 // decimation-in-frequency stage of half-size h = n / 2^(r+1) (h >= 2), entry
 // j = W_{2h}^{j mod h} where bit h of j is set, else 1.
 std::vector<double> xlane_twiddles(int64_t n);
+// Operands of the 16-point matrix-core kernel: the real 32 x 32 DFT_16
+// matrix [[Wr, -Wi], [Wi, Wr]] in the A layout of v_mfma_f32_32x32x16_f16
+// (lane l holds row l % 32, K columns 8 (l / 32) .. + 7), for the real and
+// imaginary K halves, as f16 high and low parts: 4 x 64 x 8 f16 bit patterns.
+std::vector<uint16_t> mfma_dft16_table();
 
 // Bluestein chirp c_k = exp(-i pi (k^2 mod 2n)/n), k < n, interleaved FP64.
 std::vector<double> chirp_table(int64_t n);

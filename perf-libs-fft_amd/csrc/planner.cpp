@@ -124,7 +124,7 @@ Choice choose(int kind, int64_t len, int prec, bool bwd, int need_tpb_div, bool 
   std::vector<Choice> stat;
   for (const auto &e : kernel_registry()) {
     if (e.kind != kind || e.prec != prec || e.bwd != bwd || e.n != len) continue;
-    if (e.xlane && !packed) continue;
+    if ((e.xlane || e.packed) && !packed) continue;
     if (need_tpb_div && need_tpb_div % e.tpb) continue;
     if (e.lds_bytes(len) > 160 * 1024) continue;
     Choice s;
@@ -161,6 +161,10 @@ hipError_t upload_table(DevMem &dst, const std::vector<double> &v, int prec) {
 
 hipError_t EngineData::init(const Choice &c, int64_t len) {
     if (c.e->xlane) return upload_table(tw, xlane_twiddles(len), c.e->prec);
+    if (c.e->mfma && c.e->packed) {
+      const std::vector<uint16_t> t = mfma_dft16_table();
+      return tw.upload(t.data(), t.size() * sizeof(uint16_t));
+    }
     hipError_t err = upload_table(tw, stockham_twiddles(c.chain), c.e->prec);
     if (err != hipSuccess || c.e->n) return err;
     GenericDesc d{};

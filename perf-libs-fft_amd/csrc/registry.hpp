@@ -66,6 +66,7 @@ struct KEntry {
   int waves = 0;        // > 0: compiled for at least this many waves per SIMD
   bool xlane = false;   // cross-lane kernel: packed members only, lane twiddle table
   bool mfma = false;    // first stage on the matrix cores
+  bool packed = false;  // members contiguous at distance n only
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len

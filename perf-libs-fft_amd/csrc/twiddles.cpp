@@ -85,3 +85,27 @@ std::vector<double> xlane_twiddles(int64_t n) {
 }
 
 }  // namespace plf
+
+namespace plf {
+
+std::vector<uint16_t> mfma_dft16_table() {
+  // operand o = 2 kh + part (kh: K half, part 0 high / 1 low), lane l,
+  // element i at ((o * 64) + l) * 8 + i
+  std::vector<uint16_t> t(4 * 64 * 8);
+  for (int kh = 0; kh < 2; ++kh)
+    for (int l = 0; l < 64; ++l)
+      for (int i = 0; i < 8; ++i) {
+        const int row = l % 32, h = l / 32, k = row % 16, m = 8 * h + i;
+        double cr, si;
+        unit_root((int64_t)k * m, 16, &cr, &si);  // cos a, sin a, a = 2 pi k m / 16
+        // rows 0..15: [Wr, -Wi] = [cos, sin]; rows 16..31: [Wi, Wr] = [-sin, cos]
+        const float w = (float)(row < 16 ? (kh == 0 ? cr : si) : (kh == 0 ? -si : cr));
+        const _Float16 hi = (_Float16)w;
+        const _Float16 lo = (_Float16)(w - (float)hi);
+        t[((2 * kh + 0) * 64 + l) * 8 + i] = __builtin_bit_cast(uint16_t, hi);
+        t[((2 * kh + 1) * 64 + l) * 8 + i] = __builtin_bit_cast(uint16_t, lo);
+      }
+  return t;
+}
+
+}  // namespace plf

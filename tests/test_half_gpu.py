@@ -106,3 +106,27 @@ def test_c2c_f16_matrix_core_stage(torch_cuda, orc, monkeypatch, direction):
         assert relerr(outs[name], ref) < HTOL["f16"], plan.describe()
     # the two kernels differ only in f32 rounding before the f16 store
     assert relerr(outs["_mfma"], outs["n4096_r16x16x16_t256_b1_p5_split"]) < 3e-4
+
+
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_c2c_f16_n16_matrix_core(torch_cuda, orc, monkeypatch, direction):
+    """n = 16 as one matrix-core product per 32 packed members, forced: it
+    matches the FP64 oracle and the cross-lane kernel (f32 butterflies) to
+    within output rounding, for a batch that leaves a partial group and more
+    groups than resident waves; a strided batch does not take it."""
+    torch = torch_cuda
+    n, hm = 16, 300007
+    xs, xv = _encode(rng.normal_f64(12, 2 * n * hm), "f16")
+    ref = orc.c2c_batched(xv[0::2] + 1j * xv[1::2], n, direction, hm, 1, n, 1, n, n * hm)
+    outs = {}
+    for name in ("_mfma", "_xlane"):
+        monkeypatch.setenv("PLFFT_KERNEL_FILTER", name)
+        plan = P.Plan("c2c", n, direction, "f16", howmany=hm)
+        assert name in plan.describe(), plan.describe()
+        y = _decode(_run(torch, plan, xs, 2 * n * hm), "f16")
+        outs[name] = y[0::2] + 1j * y[1::2]
+        assert relerr(outs[name], ref) < HTOL["f16"], plan.describe()
+    assert relerr(outs["_mfma"], outs["_xlane"]) < 3e-4
+    monkeypatch.delenv("PLFFT_KERNEL_FILTER")
+    plan = P.Plan("c2c", n, direction, "f16", howmany=8, idist=20, odist=20)
+    assert "_mfma" not in plan.describe()

@@ -2,7 +2,7 @@
 //
 // The reference tries Rader before Bluestein (src/planner/planner.cpp:178-179);
 // here a prime n whose n-1 has an LDS kernel runs fused Rader, other lengths
-// run fused Bluestein when the padded length M >= 2n-1 fits one LDS tile,
+// run fused Bluestein when the padded length M >= 2n-2 fits one LDS tile,
 // and the rest run a multi-kernel chirp-z around the four-step transform.
 // The convolution kernel B = DFT(b)/M is computed once at plan time in FP64
 // on the GPU with this library's own C2C plan, then rounded to the compute
@@ -276,19 +276,22 @@ std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
       return finish(std::make_unique<FusedConvNode>(true, c, p.n, p.n - 1, p.prec, p.howmany,
                                                     p.istride, id, p.ostride, od));
   }
-  // the padded length 2^k >= 2n-1 must stay far from int64 overflow (and
-  // from any length with a four-step plan)
+  // The padded length: the cyclic convolution with b_k = b_{M-k} is exact
+  // for M >= 2n-2, one less than the general 2n-1, because b is even
+  // (b_{-k} = b_k): at M = 2n-2 the two lags +-(n-1) share one slot that
+  // holds the same value. (n = 65537: M = 2^17 instead of 2^18.) It must
+  // stay far from int64 overflow and from any length with a four-step plan.
   if (p.n > (int64_t(1) << 40)) {
     *why = "length " + std::to_string(p.n) + " is beyond the chirp-z range";
     return nullptr;
   }
   // fused Bluestein: a power of two with a compiled chain, else a 7-smooth length
   int64_t P2 = 1;
-  while (P2 < 2 * p.n - 1) P2 *= 2;
+  while (P2 < 2 * p.n - 2) P2 *= 2;
   Choice c = choose(KK_BLUE, P2, p.prec, bwd);
   int64_t M = P2;
   if (!c.e || !c.e->n) {
-    int64_t S = 2 * p.n - 1;
+    int64_t S = 2 * p.n - 2;
     while (!smooth7(S)) ++S;
     Choice cs = choose(KK_BLUE, S, p.prec, bwd);
     if (cs.e) {

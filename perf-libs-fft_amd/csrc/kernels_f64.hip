@@ -10,6 +10,17 @@ void register_kernels_f64(std::vector<KEntry> &r) {
   PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 2);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 4);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 8);
+  // prime n <= 23: one thread per member, the whole odd-prime butterfly in
+  // registers (the runtime-chain kernel spends 16 threads on its one
+  // butterfly)
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 3);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 5);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 7);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 11);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 13);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 17);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 19);
+  PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 23);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    4,  64, 4, 4);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    4,  64, 8, 4);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    8,  32, 8, 8);

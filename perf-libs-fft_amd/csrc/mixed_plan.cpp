@@ -131,16 +131,17 @@ bool schedule_for(int64_t n, Schedule *s, std::string *why) {
     s->len = n;
     return true;
   }
-  for (int64_t M = 2 * n - 1; M <= kOneBlock; ++M)
+  // Bluestein's padded length: M >= 2n-2 suffices, as the chirp is even
+  for (int64_t M = 2 * n - 2; M <= kOneBlock; ++M)
     if (smooth7(M) && engine_fits(M, 256)) {
       s->kind = 2;
       s->len = M;
       return true;
     }
-  // chirp-z: among the 7-smooth M >= 2n-1 up to 1/8 above the first one
+  // chirp-z: among the 7-smooth M >= 2n-2 up to 1/8 above the first one
   // that splits, the one with the least predicted cost (three tile passes)
   int64_t first_M = 0, best_cost = 0;
-  for (int64_t M = std::max<int64_t>(2 * n - 1, kOneBlock + 1); M <= kMaxSide * kMaxSide; ++M) {
+  for (int64_t M = std::max<int64_t>(2 * n - 2, kOneBlock + 1); M <= kMaxSide * kMaxSide; ++M) {
     if (first_M && M > first_M + first_M / 8) break;
     Split sp;
     if (!smooth7(M) || !best_split(M, &sp)) continue;

@@ -28,7 +28,7 @@ enum KKind : int {
   KK_C2R_E = 4,  // C2R, even n, engine length n/2
   KK_R2C_F = 5,  // R2C, full-length engine
   KK_C2R_F = 6,  // C2R, full-length engine
-  KK_BLUE = 7,   // fused Bluestein, engine length M >= 2n-1
+  KK_BLUE = 7,   // fused Bluestein, engine length M >= 2n-2
   KK_RADER = 8,  // fused Rader, engine length n-1
   KK_DCT2_E = 9,   // DCT-II (bwd slot: DST-II), even n, engine length n/2
   KK_DCT2_F = 10,  // DCT-II / DST-II, full-length engine

@@ -251,6 +251,140 @@ class BigBluesteinNode : public Node {
   DevMem B_, c_;
 };
 
+// Chirp-z in three passes (kernels_cz.hpp) over a power-of-two M = n1 * n2
+// whose sides have compiled column and row engines: chirp-in columns, rows
+// that finish DFT_M, multiply by B and start the inverse, chirp-out columns.
+// Each member's workspace is M complex values of the compute precision.
+class ChirpZ3Node : public Node {
+ public:
+  ChirpZ3Node(int64_t n, int64_t n1, int64_t n2, int prec, bool bwd, int64_t batch, int64_t is,
+              int64_t id, int64_t os, int64_t od, int64_t chunk, Choice cin, Choice cmid,
+              Choice cout)
+      : n_(n), n1_(n1), n2_(n2), prec_(prec), bwd_(bwd), batch_(batch), is_(is), id_(id),
+        os_(os), od_(od), chunk_(chunk), cin_(std::move(cin)), cmid_(std::move(cmid)),
+        cout_(std::move(cout)) {}
+  hipError_t init() {
+    hipError_t err = ein_.init(cin_, n1_);
+    if (err == hipSuccess) err = emid_.init(cmid_, n2_);
+    if (err == hipSuccess) err = eout_.init(cout_, n1_);
+    if (err != hipSuccess) return err;
+    const int64_t M = n1_ * n2_;
+    int lg = 0;
+    while ((int64_t(1) << lg) < M) ++lg;
+    shift_ = (lg + 1) / 2;
+    const int64_t nlo = int64_t(1) << shift_, nhi = ((M - 1) >> shift_) + 1;
+    // W_M split table: lo[r] = W_M^r (r < 2^shift), then hi[q] = W_M^{q 2^shift}
+    std::vector<double> sp;
+    for (int64_t r = 0; r < nlo; ++r) {
+      double re, im;
+      unit_root(r, M, &re, &im);
+      sp.push_back(re);
+      sp.push_back(-im);
+    }
+    for (int64_t q = 0; q < nhi; ++q) {
+      double re, im;
+      unit_root(q * nlo, M, &re, &im);
+      sp.push_back(re);
+      sp.push_back(-im);
+    }
+    err = upload_table(split_, sp, prec_);
+    if (err != hipSuccess) return err;
+    std::vector<double> c = chirp_table(n_), B;
+    err = bluestein_kernel(n_, M, c, &B);
+    if (err != hipSuccess) return err;
+    // the middle pass reads row k1 of B[k1 + n1 k2]: store it row-major
+    std::vector<double> Bt(B.size());
+    for (int64_t k1 = 0; k1 < n1_; ++k1)
+      for (int64_t k2 = 0; k2 < n2_; ++k2) {
+        Bt[2 * (k1 * n2_ + k2)] = B[2 * (k1 + n1_ * k2)];
+        Bt[2 * (k1 * n2_ + k2) + 1] = B[2 * (k1 + n1_ * k2) + 1];
+      }
+    err = upload_table(B_, Bt, prec_);
+    if (err == hipSuccess) err = upload_table(c_, c, prec_);
+    return err;
+  }
+  size_t per_bytes() const { return (size_t)(n1_ * n2_) * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return (size_t)chunk_ * per_bytes(); }
+  size_t min_workspace_bytes() const override { return per_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    return run(in, out, work, chunk_, s);
+  }
+  // a workspace of `cap` bytes: members in chunks of cap / per_bytes()
+  hipError_t enqueue_capped(const void *in, void *out, void *work, size_t cap,
+                            hipStream_t s) const override {
+    const int64_t c = std::min<int64_t>(chunk_, (int64_t)(cap / per_bytes()));
+    if (c < 1) return hipErrorInvalidValue;
+    return run(in, out, work, c, s);
+  }
+  hipError_t run(const void *in, void *out, void *work, int64_t chunk, hipStream_t s) const {
+    const size_t eb = 2 * (size_t)prec_real_bytes(prec_);
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk) {
+      const int64_t cb = std::min(chunk, batch_ - b0);
+      KArgs a{};
+      a.aux0 = split_.p;
+      a.n1 = n1_;
+      a.n2 = n2_;
+      a.bigdist = n_;
+      a.shift = shift_;
+      // 1. chirp-in columns
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * eb;
+      a.out = work;
+      a.istride = is_;
+      a.idist = id_;
+      a.tw = ein_.tw.p;
+      a.desc = ein_.d();
+      a.aux1 = c_.p;
+      a.n = n1_;
+      a.count = cb * n2_;
+      hipError_t err = cin_.e->launch(a, (a.count + cin_.e->tpb - 1) / cin_.e->tpb,
+                                      cin_.e->lds_bytes(n1_), s);
+      // 2. rows: DFT_n2, x B, the inverse's first DFT_n2
+      if (err == hipSuccess) {
+        a.tw = emid_.tw.p;
+        a.desc = emid_.d();
+        a.aux1 = B_.p;
+        a.n = n2_;
+        a.count = cb * n1_;
+        err = cmid_.e->launch(a, (a.count + cmid_.e->tpb - 1) / cmid_.e->tpb,
+                              cmid_.e->lds_bytes(n2_), s);
+      }
+      // 3. chirp-out columns
+      if (err == hipSuccess) {
+        a.in = work;
+        a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * eb;
+        a.ostride = os_;
+        a.odist = od_;
+        a.tw = eout_.tw.p;
+        a.desc = eout_.d();
+        a.aux1 = c_.p;
+        a.n = n1_;
+        a.count = cb * n2_;
+        err = cout_.e->launch(a, (a.count + cout_.e->tpb - 1) / cout_.e->tpb,
+                              cout_.e->lds_bytes(n1_), s);
+      }
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(chirpz3 n=" << n_ << " M=" << n1_ * n2_ << " = " << n1_ << "x" << n2_
+      << " chunk=" << chunk_ << " batch=" << batch_ << " " << chain_str(cin_) << " "
+      << chain_str(cmid_) << " " << chain_str(cout_) << ")";
+    return o.str();
+  }
+
+ private:
+  int64_t n_, n1_, n2_;
+  int prec_;
+  bool bwd_;
+  int64_t batch_, is_, id_, os_, od_, chunk_;
+  Choice cin_, cmid_, cout_;
+  EngineData ein_, emid_, eout_;
+  int shift_ = 0;
+  DevMem split_, B_, c_;
+};
+
 static bool smooth7(int64_t v) {
   for (int p : {2, 3, 5, 7})
     while (v % p == 0) v /= p;
@@ -302,6 +436,35 @@ std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
   if (c.e)
     return finish(std::make_unique<FusedConvNode>(false, c, p.n, M, p.prec, p.howmany, p.istride,
                                                   id, p.ostride, od));
+  // chirp-z in three passes when compiled column and row engines cover a
+  // split of P2 (the most balanced, columns no longer than rows)
+  {
+    int64_t best1 = 0;
+    Choice bi, bm, bo;
+    for (int64_t n1 = 2; n1 < P2; n1 *= 2) {
+      const int64_t n2 = P2 / n1;
+      if (n1 > n2) break;
+      Choice ci = choose(KK_CZ_COL_IN, n1, p.prec, bwd);
+      Choice cm = choose(KK_CZ_ROW_MID, n2, p.prec, bwd);
+      Choice co = choose(KK_CZ_COL_OUT, n1, p.prec, bwd);
+      if (!ci.e || !cm.e || !co.e) continue;
+      best1 = n1;
+      bi = ci;
+      bm = cm;
+      bo = co;
+    }
+    if (best1) {
+      const int64_t chunk = fourstep_chunk(P2, p.prec, p.howmany);
+      auto node = std::make_unique<ChirpZ3Node>(p.n, best1, P2 / best1, p.prec, bwd, p.howmany,
+                                                p.istride, id, p.ostride, od, chunk, bi, bm, bo);
+      const hipError_t err = node->init();
+      if (err != hipSuccess) {
+        *why = std::string("chirp-z setup failed: ") + hipGetErrorString(err);
+        return nullptr;
+      }
+      return node;
+    }
+  }
   // chirp-z around the four-step transform
   const int64_t per = P2 * 2 * prec_compute_bytes(p.prec);
   const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(p.howmany, (int64_t(64) << 20) / per));

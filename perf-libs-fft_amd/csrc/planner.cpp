@@ -80,6 +80,8 @@ const std::vector<KEntry> &kernel_registry() {
     register_kernels_f32_unit(r);
     register_kernels_f32_real(r);
     register_kernels_f64(r);
+    register_kernels_f32_cz(r);
+    register_kernels_f64_cz(r);
     register_kernels_f64_real(r);
     register_kernels_f32_real4s(r);
     register_kernels_f64_real4s(r);

@@ -39,13 +39,17 @@ enum KKind : int {
   KK_ROW_R2C = 15,   //   row pass with the fused post-twiddle
   KK_COL_C2R = 16,   // C2R of even n: column pass with the fused pre-twiddle
   KK_ROW_C2R = 17,   //   row pass storing real pairs
+  KK_CZ_COL_IN = 18,   // chirp-z in three passes: chirp-in column pass
+  KK_CZ_ROW_MID = 19,  //   row pass: DFT, x B, first inverse DFT
+  KK_CZ_COL_OUT = 20,  //   chirp-out column pass
 };
 inline const char *kkind_name(int k) {
   static const char *names[] = {"c2c",       "col",       "row",       "r2c_half", "c2r_half",
                                 "r2c_full",  "c2r_full",  "bluestein", "rader",    "dct2_half",
                                 "dct2_full", "dct3_half", "dct3_full", "c2c_unit", "col_r2c",
-                                "row_r2c",   "col_c2r",   "row_c2r"};
-  return (k >= 0 && k <= 17) ? names[k] : "?";
+                                "row_r2c",   "col_c2r",   "row_c2r",  "cz_col_in", "cz_row_mid",
+                                "cz_col_out"};
+  return (k >= 0 && k <= 20) ? names[k] : "?";
 }
 
 using KLaunch = hipError_t (*)(const KArgs &, int64_t grid, size_t lds, hipStream_t);
@@ -108,6 +112,8 @@ hipError_t launch_chirp_out(int prec, bool bwd, const KArgs &a, hipStream_t s);
 // (the other R2R kinds); `total` threads.
 hipError_t launch_real_stage(int which, int prec, const KArgs &a, int64_t total, hipStream_t s);
 void register_kernels_f64(std::vector<KEntry> &r);
+void register_kernels_f32_cz(std::vector<KEntry> &r);
+void register_kernels_f64_cz(std::vector<KEntry> &r);
 void register_kernels_f16(std::vector<KEntry> &r);
 void register_kernels_bf16(std::vector<KEntry> &r);
 

@@ -157,8 +157,9 @@ def progress(msg: str) -> None:
 def bench_mixed(torch, P, steps: int, warmup: int, fused_only: bool = False):
     """The mixed-length batch: 10,000 C2C members, N log-uniform in
     [16, 65536], 10 % moved to a prime, packed back to back, forward,
-    out-of-place. Fused = one mixed plan, at most three launches (one per
-    dependency level of its tasks); plain = one ordinary plan per member,
+    out-of-place. Fused = one mixed plan: two tile launches (one per
+    dependency level) and three per group of chirp-z members of equal padded
+    length; plain = one ordinary plan per member,
     executed one after another
     (eagerly, and captured in a graph to take host launch cost out)."""
     import numpy as np

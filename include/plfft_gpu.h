@@ -81,8 +81,10 @@ const char *plfft_last_error(void);
 /* ------------------------------------------------------------------------
  * Mixed-length batches: one plan transforms `count` complex-to-complex
  * members of different lengths (powers of two, other smooth lengths and
- * primes alike) in at most three kernel launches, one per dependency level
- * of the multi-pass members. The reference has no such object: a
+ * primes alike) in a fixed number of kernel launches: at most two for the
+ * one-pass and four-step members (one per dependency level) and three per
+ * group of chirp-z members of equal padded length (at most five groups for
+ * lengths up to 65536). The reference has no such object: a
  * reference caller creates one plan per length (plfft_plan_create_from_config,
  * include/plfft.h:491-501) and executes them one after another; this plan
  * replaces that loop.
@@ -93,7 +95,7 @@ const char *plfft_last_error(void);
  * transformed in place (equal offsets, input == output); member regions must
  * not otherwise overlap. Unnormalized, like every plfft transform. Lengths up
  * to 131072 are accepted for any n, and up to 262144 for n whose prime
- * factors are all at most 23. data_type: FP32, FP64, FP16 or BF16 (FP32
+ * factors are all at most 7. data_type: FP32, FP64, FP16 or BF16 (FP32
  * arithmetic for the 16-bit types), or Q15 / Q7 with power-of-two lengths up
  * to 32768 (one launch per distinct length; other lengths fail, as the
  * reference's fixed-point plans do).

@@ -25,6 +25,7 @@ struct KArgs {
   const GenericDesc *desc;   // dynamic-engine chain (device memory) or null
   const void *aux0;          // kernel-specific tables (four-step twiddles, chirps, ...)
   const void *aux1;
+  const void *aux2;          // per-member records of a gathered launch, or null
   int64_t count;             // transforms (or columns / rows) in this launch
   int64_t istride, idist;
   int64_t ostride, odist;
@@ -34,6 +35,15 @@ struct KArgs {
   int shift;                 // four-step twiddle split: W^e = hi[e >> shift] * lo[e & mask]
   int shift2;                // second split table (real four-step: W_n)
   int flags;
+};
+
+// One member of a gathered three-pass chirp-z launch (kernels_cz.hpp):
+// offsets in elements of the input / output storage type, the member's
+// workspace (compute complex) and its chirp and B tables in aux1.
+struct CzMember {
+  int64_t n;
+  int64_t in_off, out_off, ws_off;
+  int64_t chirp_off, B_off;
 };
 
 }  // namespace plf

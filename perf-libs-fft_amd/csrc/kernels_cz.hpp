@@ -21,6 +21,9 @@
 // rows (members * n1). Pass 1: in = input (storage type, istride, idist),
 // out = workspace, aux1 = chirp. Pass 2: out = workspace, aux1 = B. Pass 3:
 // in = workspace, out = output (ostride, odist), aux1 = chirp.
+// Gathered launch (aux2 = CzMember records, one per member of the launch;
+// the members of a mixed-length batch that share M): each member's length,
+// unit-stride offsets, workspace and table offsets come from its record.
 #pragma once
 
 #include "kernels.hpp"
@@ -43,13 +46,15 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_col_in(KArgs a) {
   const int t = threadIdx.x / TPB;
   const int64_t col = (int64_t)blockIdx.x * TPB + g;
   const bool valid = col < a.count;
-  const int64_t n1 = a.n1, n2 = a.n2, nn = a.bigdist;
+  const int64_t n1 = a.n1, n2 = a.n2;
   const int64_t bi = valid ? col / n2 : 0, c2 = valid ? col % n2 : 0;
-  const E *in = static_cast<const E *>(a.in) + bi * a.idist;
-  cpx<C> *ws = static_cast<cpx<C> *>(a.out) + bi * n1 * n2 + c2;
+  const CzMember *mt = static_cast<const CzMember *>(a.aux2);
+  const int64_t nn = mt ? mt[bi].n : a.bigdist;
+  const E *in = static_cast<const E *>(a.in) + (mt ? mt[bi].in_off : bi * a.idist);
+  cpx<C> *ws = static_cast<cpx<C> *>(a.out) + (mt ? mt[bi].ws_off : bi * n1 * n2) + c2;
   const cpx<C> *split = static_cast<const cpx<C> *>(a.aux0);
-  const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1);
-  const int64_t is = a.istride;
+  const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1) + (mt ? mt[bi].chirp_off : 0);
+  const int64_t is = mt ? 1 : a.istride;
   const int shift = a.shift;
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
   Eng::template run<false>(
@@ -73,9 +78,10 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_row_mid(KArgs a) {
   const bool valid = row < a.count;
   const int64_t n1 = a.n1, n2 = a.n2;
   const int64_t bi = valid ? row / n1 : 0, k1 = valid ? row % n1 : 0;
-  cpx<C> *rowp = static_cast<cpx<C> *>(a.out) + bi * n1 * n2 + k1 * n2;
+  const CzMember *mt = static_cast<const CzMember *>(a.aux2);
+  cpx<C> *rowp = static_cast<cpx<C> *>(a.out) + (mt ? mt[bi].ws_off : bi * n1 * n2) + k1 * n2;
   const cpx<C> *split = static_cast<const cpx<C> *>(a.aux0);
-  const cpx<C> *Brow = static_cast<const cpx<C> *>(a.aux1) + k1 * n2;
+  const cpx<C> *Brow = static_cast<const cpx<C> *>(a.aux1) + (mt ? mt[bi].B_off : 0) + k1 * n2;
   const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw);
   const int shift = a.shift;
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
@@ -101,12 +107,14 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_col_out(KArgs a) {
   const int t = threadIdx.x / TPB;
   const int64_t col = (int64_t)blockIdx.x * TPB + g;
   const bool valid = col < a.count;
-  const int64_t n1 = a.n1, n2 = a.n2, nn = a.bigdist;
+  const int64_t n1 = a.n1, n2 = a.n2;
   const int64_t bi = valid ? col / n2 : 0, c2 = valid ? col % n2 : 0;
-  const cpx<C> *ws = static_cast<const cpx<C> *>(a.in) + bi * n1 * n2 + c2;
-  E *out = static_cast<E *>(a.out) + bi * a.odist;
-  const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1);
-  const int64_t os = a.ostride;
+  const CzMember *mt = static_cast<const CzMember *>(a.aux2);
+  const int64_t nn = mt ? mt[bi].n : a.bigdist;
+  const cpx<C> *ws = static_cast<const cpx<C> *>(a.in) + (mt ? mt[bi].ws_off : bi * n1 * n2) + c2;
+  E *out = static_cast<E *>(a.out) + (mt ? mt[bi].out_off : bi * a.odist);
+  const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1) + (mt ? mt[bi].chirp_off : 0);
+  const int64_t os = mt ? 1 : a.ostride;
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
   Eng::template run<false>(
       a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,

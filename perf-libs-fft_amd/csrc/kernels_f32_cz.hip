@@ -1,13 +1,17 @@
-// kernels_f32_cz.hip -- FP32 three-pass chirp-z kernels (the column and row
-// engines of the four-step passes, sides 128..1024).
+// kernels_f32_cz.hip -- three-pass chirp-z kernels with FP32 arithmetic
+// (f32, f16 and bf16 storage; the column and row engines of the four-step
+// passes, sides 64..1024).
 #include "kernels_cz.hpp"
 #include "launch.hpp"
 
 namespace plf {
 
-void register_kernels_f32_cz(std::vector<KEntry> &r) {
-  using S = float;
-  constexpr int P = P_F32;
+// sides 64..1024 with FP32 arithmetic for storage S
+template <typename S, int P>
+void register_cz_f32_compute(std::vector<KEntry> &r) {
+  PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,    8,  16, 8, 8);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,    8,  16, 8, 8);
+  PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,    8,  16, 8, 8);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,    8,  16, 16, 8);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   16,  32, 16, 16);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   32,  16, 16, 16, 2);
@@ -20,6 +24,12 @@ void register_kernels_f32_cz(std::vector<KEntry> &r) {
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   16,  32, 16, 16);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   32,  16, 16, 16, 2);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   64,  16, 16, 16, 4);
+}
+
+void register_kernels_f32_cz(std::vector<KEntry> &r) {
+  register_cz_f32_compute<float, P_F32>(r);
+  register_cz_f32_compute<half_pair, P_F16>(r);
+  register_cz_f32_compute<bf16_pair, P_BF16>(r);
 }
 
 }  // namespace plf

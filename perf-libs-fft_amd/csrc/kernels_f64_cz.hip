@@ -1,5 +1,5 @@
 // kernels_f64_cz.hip -- FP64 three-pass chirp-z kernels (the column and row
-// engines of the four-step passes, sides 128..512).
+// engines of the four-step passes, sides 64..512).
 #include "kernels_cz.hpp"
 #include "launch.hpp"
 
@@ -8,6 +8,9 @@ namespace plf {
 void register_kernels_f64_cz(std::vector<KEntry> &r) {
   using S = double;
   constexpr int P = P_F64;
+  PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,    8,   8, 8, 8);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,    8,   8, 8, 8);
+  PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,    8,   8, 8, 8);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   16,   8, 8, 4, 4);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   32,   8, 8, 8, 4);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   64,   8, 8, 8, 8);

@@ -1,4 +1,4 @@
-// kernels_mixed.hpp -- a batch of mixed lengths in at most three launches.
+// kernels_mixed.hpp -- the tile kernels of a mixed-length batch.
 //
 // Members are split into tasks of one 256-thread block each:
 //   ONE       a whole member in LDS: direct (runtime-chain engine) or
@@ -6,14 +6,12 @@
 //   COL/ROW   four-step tiles: 16 (or 8) columns per pass through LDS,
 //             twiddled into the member's workspace; then 16 (or 8) rows
 //             with a transposed store
-//   CZ_*      chirp-z (Bluestein whose DFT_M are four-step): chirp-in
-//             column tiles; row tiles that finish DFT_M, multiply by B and
-//             start the inverse DFT_M in LDS; chirp-out column tiles
-// A launch runs one dependency level (ONE + COL + CZ_IN, then ROW + CZ_MID,
-// then CZ_OUT); stream order separates the levels, so blocks never wait for
-// each other and need no atomics. Within a launch the host orders the tasks
-// by predicted cost and block b takes positions b, 2G-1-b, 2G+b, ... (snake
-// order over G blocks).
+// (chirp-z members run on the three-pass kernels of kernels_cz.hpp, in
+// groups). A launch runs one dependency level (ONE + COL, then ROW); stream
+// order separates the levels, so blocks never wait for each other and need
+// no atomics. Within a launch the host orders the tasks by predicted cost
+// and block b takes positions b, 2G-1-b, 2G+b, ... (snake order over G
+// blocks).
 #pragma once
 
 #include "engine.hpp"
@@ -31,9 +29,6 @@ struct MixedExec {
   using Eng16 = DynamicEngine<C, 16, 16, true, true>;   // 16 lanes x 16 threads
   using Eng8 = DynamicEngine<C, 32, 16, true, true>;    // 8 lanes x 32 threads
   static constexpr int PAD = 5;
-
-  // pass kinds of the tile functions
-  enum Pass : int { COL = 0, ROW = 1, CZ_IN = 2, CZ_OUT = 3 };
 
   // backward = swap o forward o swap (swap exchanges re and im)
   static __device__ __forceinline__ cpx<C> io(const MixedArgs &a, cpx<C> v) {
@@ -87,8 +82,8 @@ struct MixedExec {
   // column pass over columns [first, end) (at most LANES) of the n1 x n2
   // matrix; the host starts tiles where the member's rows are line-aligned
   template <class EngT, int LANES>
-  static __device__ __forceinline__ void col_tile(const MixedArgs &a, int pass, int first,
-                                                  int end, const MixedXf &x, cpx<C> *lds_all) {
+  static __device__ __forceinline__ void col_tile(const MixedArgs &a, int first, int end,
+                                                  const MixedXf &x, cpx<C> *lds_all) {
     const int g = threadIdx.x % LANES;
     const int t = threadIdx.x / LANES;
     const int n1 = x.n1, n2 = x.n2;
@@ -100,27 +95,8 @@ struct MixedExec {
     const cpx<C> *aux = static_cast<const cpx<C> *>(a.aux);
     cpx<C> *ws = static_cast<cpx<C> *>(a.ws) + x.ws_off;
     const E *in = static_cast<const E *>(a.in) + x.in_off;
-    const cpx<C> *chirp = aux + x.chirp_off;
-    if (pass == CZ_OUT) {
-      // last pass of chirp-z: DFT_n1 down column m2 of the workspace gives
-      // bins m = m2 + n2 * m1; unswap, multiply by the chirp, store m < n
-      E *out = static_cast<E *>(a.out) + x.out_off;
-      EngT::template run<false>(
-          d, lds, tw, t, valid, [&](int e) { return ws[col + (int64_t)e * n2]; },
-          [&](int m1, cpx<C> v) {
-            const int64_t m = col + (int64_t)n2 * m1;
-            if (m < x.n) out[m] = ST::st(io(a, cmul(cswap(v), chirp[m])));
-          });
-      return;
-    }
     EngT::template run<false>(
-        d, lds, tw, t, valid,
-        [&](int e) {
-          const int64_t j = col + (int64_t)e * n2;
-          if (pass == COL) return io(a, ST::ld(in[j]));
-          if (j >= x.n) return cpx<C>{C(0), C(0)};  // CZ_IN: chirp, zero padding
-          return cmul(io(a, ST::ld(in[j])), chirp[j]);
-        },
+        d, lds, tw, t, valid, [&](int e) { return io(a, ST::ld(in[col + (int64_t)e * n2])); },
         [&](int k1, cpx<C> v) {
           ws[(int64_t)k1 * n2 + col] = cmul(v, split_tw(aux, x, (int64_t)col * k1));
         });
@@ -153,43 +129,9 @@ struct MixedExec {
       out[rc + (int64_t)n1 * k2] = ST::st(io(a, l[lds_pad<PAD>(k2)]));
   }
 
-  // Middle pass of chirp-z, rows [first, first + LANES) of the workspace.
-  // Row k1 holds column-pass outputs; its DFT_n2 gives A[k1 + n1 k2] in LDS.
-  // The inverse transform of A B (as swap o DFT o swap) splits the same way
-  // with the roles of the sides exchanged: its first pass is a DFT_n2 over
-  // that same row, so the row is multiplied by B, swapped, transformed again
-  // in LDS, twiddled by W_M^{k1 m2} and written back in place.
-  template <class EngT, int LANES>
-  static __device__ __forceinline__ void row_mid(const MixedArgs &a, int first, const MixedXf &x,
-                                                 cpx<C> *lds_all) {
-    constexpr int TPT = 256 / LANES;
-    const int g = threadIdx.x / TPT;
-    const int t = threadIdx.x % TPT;
-    const int n1 = x.n1, n2 = x.n2;
-    const int row = first + g;
-    const bool valid = row < n1;
-    cpx<C> *lds = lds_all + g * lds_padded<PAD>(n2);
-    const GenericDesc *d = a.descs + x.d1;
-    const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw) + a.desc_tw[x.d1];
-    const cpx<C> *aux = static_cast<const cpx<C> *>(a.aux);
-    const cpx<C> *B = aux + x.B_off;
-    cpx<C> *rowp = static_cast<cpx<C> *>(a.ws) + x.ws_off + (int64_t)(valid ? row : 0) * n2;
-    EngT::template run<true>(
-        d, lds, tw, t, valid, [&](int e) { return rowp[e]; }, [&](int, cpx<C>) {});
-    if (valid)
-      for (int k2 = t; k2 < n2; k2 += TPT) {
-        const int p = lds_pad<PAD>(k2);
-        lds[p] = cswap(cmul(lds[p], B[(int64_t)row * n2 + k2]));  // B row-major by k1
-      }
-    __syncthreads();
-    EngT::template run<false, true>(
-        d, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; },
-        [&](int m2, cpx<C> v) { rowp[m2] = cmul(v, split_tw(aux, x, (int64_t)row * m2)); });
-  }
-
-  // The task types of one dependency level: 0 = ONE, COL, CZ_IN;
-  // 1 = ROW, CZ_MID; 2 = CZ_OUT. One kernel per level keeps each kernel's
-  // code to the engine instances that level runs.
+  // The task types of one dependency level: 0 = ONE, COL; 1 = ROW. One
+  // kernel per level keeps each kernel's code to the engine instances that
+  // level runs.
   template <int LEVEL>
   static __device__ __forceinline__ void run(const MixedArgs &a, cpx<C> *lds) {
     const int G = gridDim.x, b = blockIdx.x;
@@ -202,21 +144,12 @@ struct MixedExec {
         if (tk.type == MT_ONE) {
           one_block(a, x, lds);
         } else {
-          const int pass = tk.type == MT_COL ? COL : CZ_IN;
-          if (x.lanes0 == 16) col_tile<Eng16, 16>(a, pass, tk.first, tk.end, x, lds);
-          else col_tile<Eng8, 8>(a, pass, tk.first, tk.end, x, lds);
+          if (x.lanes0 == 16) col_tile<Eng16, 16>(a, tk.first, tk.end, x, lds);
+          else col_tile<Eng8, 8>(a, tk.first, tk.end, x, lds);
         }
-      } else if constexpr (LEVEL == 1) {
-        if (tk.type == MT_ROW) {
-          if (x.lanes1 == 16) row_tile<Eng16, 16>(a, tk.first, x, lds);
-          else row_tile<Eng8, 8>(a, tk.first, x, lds);
-        } else {  // MT_CZ_MID
-          if (x.lanes1 == 16) row_mid<Eng16, 16>(a, tk.first, x, lds);
-          else row_mid<Eng8, 8>(a, tk.first, x, lds);
-        }
-      } else {  // MT_CZ_OUT
-        if (x.lanes0 == 16) col_tile<Eng16, 16>(a, CZ_OUT, tk.first, tk.end, x, lds);
-        else col_tile<Eng8, 8>(a, CZ_OUT, tk.first, tk.end, x, lds);
+      } else {  // MT_ROW
+        if (x.lanes1 == 16) row_tile<Eng16, 16>(a, tk.first, x, lds);
+        else row_tile<Eng8, 8>(a, tk.first, x, lds);
       }
       __syncthreads();  // the next task rewrites the LDS this one read
     }

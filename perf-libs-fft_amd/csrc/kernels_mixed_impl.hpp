@@ -23,7 +23,6 @@ hipError_t launch_mixed_t(int level, const MixedArgs &a, int grid, size_t lds, h
   switch (level) {
     case 0: return launch_mixed_level<S, 0>(a, grid, lds, s);
     case 1: return launch_mixed_level<S, 1>(a, grid, lds, s);
-    case 2: return launch_mixed_level<S, 2>(a, grid, lds, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -47,7 +46,6 @@ int mixed_blocks_per_cu_t(int level, size_t lds) {
   switch (level) {
     case 0: return mixed_blocks_level<S, 0>(lds);
     case 1: return mixed_blocks_level<S, 1>(lds);
-    case 2: return mixed_blocks_level<S, 2>(lds);
     default: return 0;
   }
 }

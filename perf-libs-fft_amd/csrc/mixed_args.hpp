@@ -12,26 +12,23 @@ namespace plf {
 // Per-member description (device memory).
 struct MixedXf {
   int n;         // transform length
-  int kind;      // 0 direct, 1 four-step, 2 Bluestein in one block, 3 chirp-z four-step
+  int kind;      // 0 direct, 1 four-step, 2 Bluestein in one block, 3 chirp-z (grouped, no tiles)
   int len;       // engine length of a one-block member (n or M); M for chirp-z
-  int n1, n2;    // four-step split of n (kind 1) or of M (kind 3)
+  int n1, n2;    // four-step split of n (kind 1)
   int d0, d1;    // engine descriptors: one-block (d0) or column (d0) / row (d1)
   int shift;     // W_N split: N = n1*n2, W^e = hi[e >> shift] * lo[e & mask]
   int lanes0, lanes1;  // tile width of the column / row passes (16 or 8)
   int64_t in_off, out_off;  // complex elements
-  int64_t ws_off;           // complex elements of workspace (kind 1: N; kind 3: M)
+  int64_t ws_off;           // complex elements of workspace (kind 1: n)
   int64_t tw4_off;          // split twiddles in aux: lo (2^shift), then hi
   int64_t chirp_off;        // chirp c_k (n entries) in aux, at the residue of in_off mod 16
   int64_t B_off;            // B = DFT_M(b)/M (M entries) in aux; chirp-z: row-major n1 x n2
 };
 
 enum MixedTaskType : int {
-  MT_ONE = 0,     // whole member in one pass through LDS
-  MT_COL = 1,     // four-step column tile
-  MT_ROW = 2,     // four-step row tile
-  MT_CZ_IN = 3,   // chirp-z: chirp-in column tile
-  MT_CZ_MID = 4,  // chirp-z: DFT, x B, inverse-DFT row tile
-  MT_CZ_OUT = 5,  // chirp-z: chirp-out column tile
+  MT_ONE = 0,  // whole member in one pass through LDS
+  MT_COL = 1,  // four-step column tile
+  MT_ROW = 2,  // four-step row tile
 };
 
 struct MixedTask {
@@ -55,7 +52,7 @@ struct MixedArgs {
   int bwd;                 // 1: backward transform (exponent sign +)
 };
 
-// Launch the mixed-batch kernel of dependency level `level` (0, 1, 2) for
+// Launch the mixed-batch kernel of dependency level `level` (0, 1) for
 // storage precision `prec` (Prec values).
 hipError_t launch_mixed(int prec, int level, const MixedArgs &a, int grid, size_t lds,
                         hipStream_t s);

@@ -115,12 +115,33 @@ bool best_split(int64_t N, Split *out) {
 }
 
 struct Schedule {
-  int kind = -1;  // 0 direct, 1 four-step, 2 Bluestein one block, 3 chirp-z
+  int kind = -1;  // 0 direct, 1 four-step, 2 Bluestein one block,
+                  // 3 chirp-z on the three-pass compiled kernels
   int64_t len = 0;  // one-block engine length (n or M); M for chirp-z
   Split sp;
 };
 
-bool schedule_for(int64_t n, Schedule *s, std::string *why) {
+// Chirp-z: the power-of-two M >= 2n-2 (Bluestein's padded length; the chirp
+// is even, so 2n-2 suffices) with compiled three-pass kernels for a split
+// n1 x n2, the most balanced with n1 <= n2.
+bool cz_split(int64_t n, int prec, bool bwd, int64_t *M, int64_t *n1) {
+  int64_t P2 = 1;
+  while (P2 < 2 * n - 2) P2 *= 2;
+  int64_t best = 0;
+  for (int64_t a = 2; a < P2; a *= 2) {
+    const int64_t b = P2 / a;
+    if (a > b) break;
+    if (choose(KK_CZ_COL_IN, a, prec, bwd).e && choose(KK_CZ_ROW_MID, b, prec, bwd).e &&
+        choose(KK_CZ_COL_OUT, a, prec, bwd).e)
+      best = a;
+  }
+  if (!best) return false;
+  *M = P2;
+  *n1 = best;
+  return true;
+}
+
+bool schedule_for(int64_t n, int prec, bool bwd, Schedule *s, std::string *why) {
   if (n <= kOneBlock && smooth(n) && engine_fits(n, 256)) {
     s->kind = 0;
     s->len = n;
@@ -131,30 +152,19 @@ bool schedule_for(int64_t n, Schedule *s, std::string *why) {
     s->len = n;
     return true;
   }
-  // Bluestein's padded length: M >= 2n-2 suffices, as the chirp is even
   for (int64_t M = 2 * n - 2; M <= kOneBlock; ++M)
     if (smooth7(M) && engine_fits(M, 256)) {
       s->kind = 2;
       s->len = M;
       return true;
     }
-  // chirp-z: among the 7-smooth M >= 2n-2 up to 1/8 above the first one
-  // that splits, the one with the least predicted cost (three tile passes)
-  int64_t first_M = 0, best_cost = 0;
-  for (int64_t M = std::max<int64_t>(2 * n - 2, kOneBlock + 1); M <= kMaxSide * kMaxSide; ++M) {
-    if (first_M && M > first_M + first_M / 8) break;
-    Split sp;
-    if (!smooth7(M) || !best_split(M, &sp)) continue;
-    if (!first_M) first_M = M;
-    const int64_t cost = split_cost(sp) + sp.n1 * sp.n2;  // + the middle pass's second DFT
-    if (s->kind != 3 || cost < best_cost) {
-      s->kind = 3;
-      s->len = M;
-      s->sp = sp;
-      best_cost = cost;
-    }
+  int64_t M = 0, n1 = 0;
+  if (cz_split(n, prec, bwd, &M, &n1)) {
+    s->kind = 3;
+    s->len = M;
+    s->sp = Split{n1, M / n1, 0, 0};
+    return true;
   }
-  if (s->kind == 3) return true;
   std::ostringstream o;
   o << "mixed batch: no schedule for length " << n;
   *why = o.str();
@@ -203,7 +213,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
   for (int64_t i = 0; i < count; ++i) {
     if (sched.count(n[i])) continue;
     Schedule s;
-    if (!schedule_for(n[i], &s, why)) return nullptr;
+    if (!schedule_for(n[i], prec, dir > 0, &s, why)) return nullptr;
     sched[n[i]] = s;
   }
 
@@ -307,7 +317,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
       const double *src = b.data() + 2 * (size_t)M * j;
       align_aux(0);
       B_of[ns[j]] = (int64_t)(aux.size() / 2);
-      if (sj.kind == 3) {
+      if (sj.kind >= 3) {
         // chirp-z: row k1 of the middle pass reads B[k1 + n1 k2], k2 < n2,
         // so B is stored row-major by k1
         const int64_t n1 = sj.sp.n1, n2 = sj.sp.n2;
@@ -340,10 +350,11 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     MixedTask t;
     double cost;
   };
-  std::vector<HostTask> level[3];
+  std::vector<HostTask> level[2];
   int64_t ws = 0, ipos = 0, opos = 0;
   size_t lds = 0;
   int nkind[4] = {0, 0, 0, 0};
+  std::map<int64_t, std::vector<CzMember>> cz_members;  // by M
   for (int64_t i = 0; i < count; ++i) {
     const int64_t nn = n[i];
     const Schedule &s = sched[nn];
@@ -373,6 +384,10 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
       level[0].push_back({MixedTask{MT_ONE, xi, 0, 0}, 2.0 * (double)nn + lwork});
       continue;
     }
+    if (s.kind == 3) {
+      cz_members[s.len].push_back(CzMember{nn, x.in_off, x.out_off, 0, x.chirp_off, x.B_off});
+      continue;
+    }
     const Split &sp = s.sp;
     x.n1 = (int)sp.n1;
     x.n2 = (int)sp.n2;
@@ -390,22 +405,51 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     lds = std::max(lds, (size_t)sp.l1 * padded(sp.n2) * csize);
     const double ccost = sp.l0 * (double)sp.n1 * (2.0 + 0.1 * std::log2((double)sp.n1));
     const double rcost = sp.l1 * (double)sp.n2 * (2.0 + 0.1 * std::log2((double)sp.n2));
-    const bool cz = s.kind == 3;
     // column tiles start where in_off + column is a multiple of the tile
     // width (a short head tile first)
     const int64_t head = (sp.l0 - r16 % sp.l0) % sp.l0;
     for (int64_t c = 0; c < sp.n2;) {
       const int64_t e = std::min<int64_t>(sp.n2, c < head ? head : c + sp.l0);
-      level[0].push_back({MixedTask{cz ? MT_CZ_IN : MT_COL, xi, (int)c, (int)e}, ccost});
-      if (cz) level[2].push_back({MixedTask{MT_CZ_OUT, xi, (int)c, (int)e}, ccost});
+      level[0].push_back({MixedTask{MT_COL, xi, (int)c, (int)e}, ccost});
       c = e;
     }
     for (int64_t r = 0; r < sp.n1; r += sp.l1)
-      level[1].push_back(
-          {MixedTask{cz ? MT_CZ_MID : MT_ROW, xi, (int)r, (int)sp.n1}, cz ? 2 * rcost : rcost});
+      level[1].push_back({MixedTask{MT_ROW, xi, (int)r, (int)sp.n1}, rcost});
   }
+  // chirp-z groups: workspace regions (line-aligned, M apart), the W_M
+  // split table in aux, the compiled engines' twiddles, member records
+  for (auto &kv : cz_members) {
+    const int64_t M = kv.first;
+    const Schedule &s0 = sched[kv.second[0].n];
+    auto g = std::make_unique<CzGroup>();
+    g->n1 = s0.sp.n1;
+    g->n2 = s0.sp.n2;
+    g->count = (int64_t)kv.second.size();
+    g->cin = choose(KK_CZ_COL_IN, g->n1, prec, dir > 0);
+    g->cmid = choose(KK_CZ_ROW_MID, g->n2, prec, dir > 0);
+    g->cout = choose(KK_CZ_COL_OUT, g->n1, prec, dir > 0);
+    ws += (16 - ws % 16) % 16;
+    for (CzMember &m : kv.second) {
+      m.ws_off = ws;
+      ws += M;
+    }
+    const auto spl = get_split(M);
+    g->split_off = spl.first;
+    g->shift = spl.second;
+    hipError_t err = g->ein.init(g->cin, g->n1);
+    if (err == hipSuccess) err = g->emid.init(g->cmid, g->n2);
+    if (err == hipSuccess) err = g->eout.init(g->cout, g->n1);
+    if (err == hipSuccess)
+      err = g->members.upload(kv.second.data(), kv.second.size() * sizeof(CzMember));
+    if (err != hipSuccess) {
+      *why = std::string("mixed batch: chirp-z group setup failed: ") + hipGetErrorString(err);
+      return nullptr;
+    }
+    P.cz_groups_.push_back(std::move(g));
+  }
+
   std::vector<MixedTask> tasks;
-  for (int L = 0; L < 3; ++L) {
+  for (int L = 0; L < 2; ++L) {
     std::stable_sort(level[L].begin(), level[L].end(),
                      [](const HostTask &a, const HostTask &b) { return a.cost > b.cost; });
     P.level_first_[L] = (int64_t)tasks.size();
@@ -430,7 +474,7 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus <= 0)
     cus = 256;
-  for (int L = 0; L < 3; ++L) {
+  for (int L = 0; L < 2; ++L) {
     const int per_cu = mixed_blocks_per_cu(prec, L, P.lds_);
     if (per_cu <= 0) {
       *why = "mixed batch: the kernel does not fit a CU with this LDS size";
@@ -442,8 +486,9 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
 
   std::ostringstream o;
   o << "(mixed count=" << count << " direct=" << nkind[0] << " fourstep=" << nkind[1]
-    << " bluestein=" << nkind[2] << " chirpz=" << nkind[3] << " lengths=" << sched.size()
-    << " tasks=" << P.level_count_[0] << "+" << P.level_count_[1] << "+" << P.level_count_[2]
+    << " bluestein=" << nkind[2] << " chirpz=" << nkind[3] << " (" << P.cz_groups_.size()
+    << " groups) lengths=" << sched.size() << " tasks=" << P.level_count_[0] << "+"
+    << P.level_count_[1]
     << " grid=" << P.grid_ << " lds=" << P.lds_
     << " workspace=" << P.workspace_bytes() << ")";
   P.summary_ = o.str();
@@ -468,12 +513,49 @@ hipError_t MixedPlan::enqueue(const void *in, void *out, void *ws, hipStream_t s
   a.tw = tw_.p;
   a.aux = aux_.p;
   a.bwd = bwd_;
-  for (int L = 0; L < 3; ++L) {
+  for (int L = 0; L < 2; ++L) {
     if (level_count_[L] == 0) continue;
     a.tasks = static_cast<const MixedTask *>(tasks_.p) + level_first_[L];
     a.ntasks = level_count_[L];
     const int grid = (int)std::min<int64_t>(level_count_[L], level_grid_[L]);
     const hipError_t err = launch_mixed(prec_, L, a, grid, lds_, s);
+    if (err != hipSuccess) return err;
+  }
+  const size_t cb = 2 * (size_t)prec_compute_bytes(prec_);
+  for (const auto &g : cz_groups_) {
+    KArgs k{};
+    k.aux0 = static_cast<const char *>(aux_.p) + (size_t)g->split_off * cb;
+    k.aux2 = g->members.p;
+    k.n1 = g->n1;
+    k.n2 = g->n2;
+    k.shift = g->shift;
+    // 1. chirp-in columns
+    k.in = in;
+    k.out = ws;
+    k.aux1 = aux_.p;
+    k.tw = g->ein.tw.p;
+    k.n = g->n1;
+    k.count = g->count * g->n2;
+    hipError_t err = g->cin.e->launch(k, (k.count + g->cin.e->tpb - 1) / g->cin.e->tpb,
+                                      g->cin.e->lds_bytes(g->n1), s);
+    // 2. rows (B tables in aux at the members' B offsets)
+    if (err == hipSuccess) {
+      k.tw = g->emid.tw.p;
+      k.n = g->n2;
+      k.count = g->count * g->n1;
+      err = g->cmid.e->launch(k, (k.count + g->cmid.e->tpb - 1) / g->cmid.e->tpb,
+                              g->cmid.e->lds_bytes(g->n2), s);
+    }
+    // 3. chirp-out columns
+    if (err == hipSuccess) {
+      k.in = ws;
+      k.out = out;
+      k.tw = g->eout.tw.p;
+      k.n = g->n1;
+      k.count = g->count * g->n2;
+      err = g->cout.e->launch(k, (k.count + g->cout.e->tpb - 1) / g->cout.e->tpb,
+                              g->cout.e->lds_bytes(g->n1), s);
+    }
     if (err != hipSuccess) return err;
   }
   return hipSuccess;

@@ -1,5 +1,6 @@
 // mixed_plan.hpp -- plan of a batch of C2C transforms of different lengths,
-// run in at most three launches (one per dependency level).
+// run as at most two tile launches (one per dependency level) plus three
+// launches per group of chirp-z members.
 #pragma once
 
 #include <memory>
@@ -22,9 +23,9 @@ class MixedPlan {
                                           const int64_t *n, const int64_t *in_off,
                                           const int64_t *out_off, std::string *why);
 
-  // Launch the kernel on `s` once per non-empty dependency level (at most
-  // three launches, nothing else). `ws` holds workspace_bytes() of device
-  // memory.
+  // Launch the tile kernel on `s` once per non-empty dependency level (at
+  // most two launches), then three launches per chirp-z group. `ws` holds
+  // workspace_bytes() of device memory.
   hipError_t enqueue(const void *in, void *out, void *ws, hipStream_t s) const;
   size_t workspace_bytes() const { return data_bytes_; }
   std::string describe() const { return summary_; }
@@ -41,10 +42,20 @@ class MixedPlan {
   int64_t count_ = 0;
   int64_t in_span_ = 0, out_span_ = 0;
   int grid_ = 0;  // resident blocks (largest grid of a launch)
-  int level_grid_[3] = {0, 0, 0};  // resident blocks of each level's kernel
-  int64_t level_first_[3] = {0, 0, 0}, level_count_[3] = {0, 0, 0};
+  int level_grid_[2] = {0, 0};  // resident blocks of each level's kernel
+  int64_t level_first_[2] = {0, 0}, level_count_[2] = {0, 0};
   size_t lds_ = 0, data_bytes_ = 0;
   DevMem xf_, tasks_, descs_, desc_tw_, tw_, aux_;
+  // chirp-z members on the three-pass compiled kernels (kernels_cz.hpp),
+  // one group per padded length M: three launches each
+  struct CzGroup {
+    Choice cin, cmid, cout;
+    EngineData ein, emid, eout;
+    int64_t n1 = 0, n2 = 0, count = 0, split_off = 0;
+    int shift = 0;
+    DevMem members;  // CzMember records
+  };
+  std::vector<std::unique_ptr<CzGroup>> cz_groups_;
   std::vector<std::unique_ptr<Node>> fixed_groups_;  // fixed point: one per length
   std::string summary_;
 };

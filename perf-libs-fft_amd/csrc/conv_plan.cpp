@@ -385,6 +385,36 @@ class ChirpZ3Node : public Node {
   DevMem split_, B_, c_;
 };
 
+bool cz3_choice(int64_t n, int prec, bool bwd, Cz3Choice *out) {
+  // candidate padded lengths M >= 2n-2 in increasing order: 2^k and 3 * 2^k
+  int64_t P2 = 1;
+  while (P2 < 2 * n - 2) P2 *= 2;
+  int64_t T3 = 3;
+  while (T3 < 2 * n - 2) T3 *= 2;
+  const int64_t cands[2] = {std::min(P2, T3), std::max(P2, T3)};
+  for (int64_t M : cands) {
+    bool found = false;
+    // column sides are powers of two (the most balanced split, n1 <= n2)
+    for (int64_t n1 = 2; n1 < M; n1 *= 2) {
+      if (M % n1) break;
+      const int64_t n2 = M / n1;
+      if (n1 > n2) break;
+      Choice ci = choose(KK_CZ_COL_IN, n1, prec, bwd);
+      Choice cm = choose(KK_CZ_ROW_MID, n2, prec, bwd);
+      Choice co = choose(KK_CZ_COL_OUT, n1, prec, bwd);
+      if (!ci.e || !cm.e || !co.e) continue;
+      out->M = M;
+      out->n1 = n1;
+      out->cin = ci;
+      out->cmid = cm;
+      out->cout = co;
+      found = true;
+    }
+    if (found) return true;
+  }
+  return false;
+}
+
 static bool smooth7(int64_t v) {
   for (int p : {2, 3, 5, 7})
     while (v % p == 0) v /= p;
@@ -437,26 +467,14 @@ std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
     return finish(std::make_unique<FusedConvNode>(false, c, p.n, M, p.prec, p.howmany, p.istride,
                                                   id, p.ostride, od));
   // chirp-z in three passes when compiled column and row engines cover a
-  // split of P2 (the most balanced, columns no longer than rows)
+  // split of a padded length 2^k or 3 * 2^k
   {
-    int64_t best1 = 0;
-    Choice bi, bm, bo;
-    for (int64_t n1 = 2; n1 < P2; n1 *= 2) {
-      const int64_t n2 = P2 / n1;
-      if (n1 > n2) break;
-      Choice ci = choose(KK_CZ_COL_IN, n1, p.prec, bwd);
-      Choice cm = choose(KK_CZ_ROW_MID, n2, p.prec, bwd);
-      Choice co = choose(KK_CZ_COL_OUT, n1, p.prec, bwd);
-      if (!ci.e || !cm.e || !co.e) continue;
-      best1 = n1;
-      bi = ci;
-      bm = cm;
-      bo = co;
-    }
-    if (best1) {
-      const int64_t chunk = fourstep_chunk(P2, p.prec, p.howmany);
-      auto node = std::make_unique<ChirpZ3Node>(p.n, best1, P2 / best1, p.prec, bwd, p.howmany,
-                                                p.istride, id, p.ostride, od, chunk, bi, bm, bo);
+    Cz3Choice cz;
+    if (cz3_choice(p.n, p.prec, bwd, &cz)) {
+      const int64_t chunk = fourstep_chunk(cz.M, p.prec, p.howmany);
+      auto node = std::make_unique<ChirpZ3Node>(p.n, cz.n1, cz.M / cz.n1, p.prec, bwd,
+                                                p.howmany, p.istride, id, p.ostride, od, chunk,
+                                                cz.cin, cz.cmid, cz.cout);
       const hipError_t err = node->init();
       if (err != hipSuccess) {
         *why = std::string("chirp-z setup failed: ") + hipGetErrorString(err);

@@ -24,6 +24,11 @@ void register_cz_f32_compute(std::vector<KEntry> &r) {
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   16,  32, 16, 16);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   32,  16, 16, 16, 2);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   64,  16, 16, 16, 4);
+  // row sides with a factor 3, for M = 3 * 2^k
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,    8,  16, 8, 4, 3);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   16,  16, 16, 4, 3);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   32,   8, 16, 8, 3);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   64,   4, 16, 16, 3);
 }
 
 void register_kernels_f32_cz(std::vector<KEntry> &r) {

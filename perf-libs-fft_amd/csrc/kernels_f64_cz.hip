@@ -20,6 +20,10 @@ void register_kernels_f64_cz(std::vector<KEntry> &r) {
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   16,   8, 8, 4, 4);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   32,   8, 8, 8, 4);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   64,   8, 8, 8, 8);
+  // row sides with a factor 3, for M = 3 * 2^k
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   16,   8, 8, 4, 3);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   32,   8, 8, 8, 3);
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   64,   4, 8, 8, 2, 3);
 }
 
 }  // namespace plf

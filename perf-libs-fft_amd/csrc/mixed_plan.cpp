@@ -121,26 +121,6 @@ struct Schedule {
   Split sp;
 };
 
-// Chirp-z: the power-of-two M >= 2n-2 (Bluestein's padded length; the chirp
-// is even, so 2n-2 suffices) with compiled three-pass kernels for a split
-// n1 x n2, the most balanced with n1 <= n2.
-bool cz_split(int64_t n, int prec, bool bwd, int64_t *M, int64_t *n1) {
-  int64_t P2 = 1;
-  while (P2 < 2 * n - 2) P2 *= 2;
-  int64_t best = 0;
-  for (int64_t a = 2; a < P2; a *= 2) {
-    const int64_t b = P2 / a;
-    if (a > b) break;
-    if (choose(KK_CZ_COL_IN, a, prec, bwd).e && choose(KK_CZ_ROW_MID, b, prec, bwd).e &&
-        choose(KK_CZ_COL_OUT, a, prec, bwd).e)
-      best = a;
-  }
-  if (!best) return false;
-  *M = P2;
-  *n1 = best;
-  return true;
-}
-
 bool schedule_for(int64_t n, int prec, bool bwd, Schedule *s, std::string *why) {
   if (n <= kOneBlock && smooth(n) && engine_fits(n, 256)) {
     s->kind = 0;
@@ -158,11 +138,12 @@ bool schedule_for(int64_t n, int prec, bool bwd, Schedule *s, std::string *why) 
       s->len = M;
       return true;
     }
-  int64_t M = 0, n1 = 0;
-  if (cz_split(n, prec, bwd, &M, &n1)) {
+  // chirp-z on the three-pass compiled kernels (padded length 2^k or 3 2^k)
+  Cz3Choice cz;
+  if (cz3_choice(n, prec, bwd, &cz)) {
     s->kind = 3;
-    s->len = M;
-    s->sp = Split{n1, M / n1, 0, 0};
+    s->len = cz.M;
+    s->sp = Split{cz.n1, cz.M / cz.n1, 0, 0};
     return true;
   }
   std::ostringstream o;

@@ -143,6 +143,15 @@ std::vector<double> split_root_table(int64_t n, int64_t count, int *shift);
 // workspace holds that many; about 1 GiB unless PLFFT_FOURSTEP_CHUNK_MB says).
 int64_t fourstep_chunk(int64_t n, int prec, int64_t batch);
 
+// Three-pass chirp-z (kernels_cz.hpp) for length n: the smallest padded
+// length M >= 2n-2 of the form 2^k or 3 * 2^k with compiled kernels for a
+// split M = n1 * n2 (n1 a power of two, n1 <= n2, the most balanced).
+struct Cz3Choice {
+  int64_t M = 0, n1 = 0;
+  Choice cin, cmid, cout;
+};
+bool cz3_choice(int64_t n, int prec, bool bwd, Cz3Choice *out);
+
 // Twiddle table of a Stockham radix chain, FORWARD sign, FP64:
 // for stage s >= 1 with radix R and L = product of earlier radices,
 // tw[off_s + (m-1)*L + j] = exp(-2*pi*i*j*m/(L*R)), m in [1,R), j in [0,L).

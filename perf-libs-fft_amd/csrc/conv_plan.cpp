@@ -2,13 +2,16 @@
 //
 // The reference tries Rader before Bluestein (src/planner/planner.cpp:178-179);
 // here a prime n whose n-1 has an LDS kernel runs fused Rader, other lengths
-// run fused Bluestein when the padded length M >= 2n-2 fits one LDS tile,
-// and the rest run a multi-kernel chirp-z around the four-step transform.
+// run fused Bluestein when the padded length M >= 2n-2 fits one LDS tile, a
+// prime whose n-1 splits into compiled four-step sides runs Rader in three
+// passes (65537: N = 2^16), and the rest run chirp-z in three passes or
+// around the four-step transform.
 // The convolution kernel B = DFT(b)/M is computed once at plan time in FP64
 // on the GPU with this library's own C2C plan, then rounded to the compute
 // precision (the reference likewise transforms b with its own plan,
 // bluestein.hpp:100-110, rader.hpp:79-90).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <sstream>
 
@@ -385,6 +388,185 @@ class ChirpZ3Node : public Node {
   DevMem split_, B_, c_;
 };
 
+// Rader for a prime n in three passes over N = n - 1 = n1 * n2
+// (rader.hpp:51-90 restated on the four-step passes): permuted-input
+// columns, the chirp-z row pass (DFT_n2, x B, the inverse's first DFT_n2,
+// plus A_0 kept), permuted-output columns adding x_0. Each member's
+// workspace is N complex values, then two slots per member of the chunk.
+class Rader3Node : public Node {
+ public:
+  Rader3Node(int64_t n, int64_t n1, int64_t n2, int prec, bool bwd, int64_t batch, int64_t is,
+             int64_t id, int64_t os, int64_t od, int64_t chunk, Choice cin, Choice cmid,
+             Choice cout)
+      : n_(n), n1_(n1), n2_(n2), prec_(prec), bwd_(bwd), batch_(batch), is_(is), id_(id),
+        os_(os), od_(od), chunk_(chunk), cin_(std::move(cin)), cmid_(std::move(cmid)),
+        cout_(std::move(cout)) {}
+  hipError_t init() {
+    hipError_t err = ein_.init(cin_, n1_);
+    if (err == hipSuccess) err = emid_.init(cmid_, n2_);
+    if (err == hipSuccess) err = eout_.init(cout_, n1_);
+    if (err != hipSuccess) return err;
+    const int64_t N = n1_ * n2_;
+    int lg = 0;
+    while ((int64_t(1) << lg) < N) ++lg;
+    shift_ = (lg + 1) / 2;
+    const int64_t nlo = int64_t(1) << shift_, nhi = ((N - 1) >> shift_) + 1;
+    // W_N split table: lo[r] = W_N^r (r < 2^shift), then hi[q] = W_N^{q 2^shift}
+    std::vector<double> sp;
+    for (int64_t r = 0; r < nlo; ++r) {
+      double re, im;
+      unit_root(r, N, &re, &im);
+      sp.push_back(re);
+      sp.push_back(-im);
+    }
+    for (int64_t q = 0; q < nhi; ++q) {
+      double re, im;
+      unit_root(q * nlo, N, &re, &im);
+      sp.push_back(re);
+      sp.push_back(-im);
+    }
+    err = upload_table(split_, sp, prec_);
+    if (err != hipSuccess) return err;
+    gen_ = group_generator(n_);
+    if (!gen_) return hipErrorInvalidValue;
+    std::vector<int> perm((size_t)N);
+    std::vector<double> b(2 * (size_t)N);
+    int64_t x = 1;
+    for (int64_t k = 0; k < N; ++k) {
+      perm[(size_t)k] = (int)x;
+      double re, im;
+      unit_root(x, n_, &re, &im);  // W_n^{perm[k]}, forward sign
+      b[2 * ((N - k) % N)] = re / (double)N;
+      b[2 * ((N - k) % N) + 1] = -im / (double)N;
+      x = x * gen_ % n_;
+    }
+    err = dft_f64_gpu(b, N, 1);
+    if (err != hipSuccess) return err;
+    // the row pass reads row k1 of B[k1 + n1 k2]: store it row-major
+    std::vector<double> Bt(b.size());
+    for (int64_t k1 = 0; k1 < n1_; ++k1)
+      for (int64_t k2 = 0; k2 < n2_; ++k2) {
+        Bt[2 * (k1 * n2_ + k2)] = b[2 * (k1 + n1_ * k2)];
+        Bt[2 * (k1 * n2_ + k2) + 1] = b[2 * (k1 + n1_ * k2) + 1];
+      }
+    err = upload_table(B_, Bt, prec_);
+    if (err == hipSuccess) err = perm_.upload(perm.data(), perm.size() * sizeof(int));
+    return err;
+  }
+  size_t per_bytes() const { return (size_t)(n1_ * n2_ + 2) * 2 * prec_compute_bytes(prec_); }
+  size_t workspace_bytes() const override { return (size_t)chunk_ * per_bytes(); }
+  size_t min_workspace_bytes() const override { return per_bytes(); }
+  hipError_t enqueue(const void *in, void *out, void *work, hipStream_t s) const override {
+    return run(in, out, work, chunk_, s);
+  }
+  hipError_t enqueue_capped(const void *in, void *out, void *work, size_t cap,
+                            hipStream_t s) const override {
+    const int64_t c = std::min<int64_t>(chunk_, (int64_t)(cap / per_bytes()));
+    if (c < 1) return hipErrorInvalidValue;
+    return run(in, out, work, c, s);
+  }
+  hipError_t run(const void *in, void *out, void *work, int64_t chunk, hipStream_t s) const {
+    const size_t eb = 2 * (size_t)prec_real_bytes(prec_);
+    const size_t cb = 2 * (size_t)prec_compute_bytes(prec_);
+    const int64_t N = n1_ * n2_;
+    // the chunk's members' slots follow their workspaces
+    void *slots = static_cast<char *>(work) + (size_t)chunk * (size_t)N * cb;
+    for (int64_t b0 = 0; b0 < batch_; b0 += chunk) {
+      const int64_t nb = std::min(chunk, batch_ - b0);
+      KArgs a{};
+      a.aux0 = split_.p;
+      a.aux1 = perm_.p;
+      a.aux3 = slots;
+      a.n1 = n1_;
+      a.n2 = n2_;
+      a.shift = shift_;
+      // 1. permuted-input columns
+      a.in = static_cast<const char *>(in) + (size_t)(b0 * id_) * eb;
+      a.out = work;
+      a.istride = is_;
+      a.idist = id_;
+      a.tw = ein_.tw.p;
+      a.desc = ein_.d();
+      a.n = n1_;
+      a.count = nb * n2_;
+      hipError_t err = cin_.e->launch(a, (a.count + cin_.e->tpb - 1) / cin_.e->tpb,
+                                      cin_.e->lds_bytes(n1_), s);
+      // 2. rows: DFT_n2, x B, the inverse's first DFT_n2 (A_0 kept)
+      if (err == hipSuccess) {
+        a.in = nullptr;
+        a.tw = emid_.tw.p;
+        a.desc = emid_.d();
+        a.aux1 = B_.p;
+        a.n = n2_;
+        a.count = nb * n1_;
+        err = cmid_.e->launch(a, (a.count + cmid_.e->tpb - 1) / cmid_.e->tpb,
+                              cmid_.e->lds_bytes(n2_), s);
+      }
+      // 3. permuted-output columns, one block per CU: the scattered stores
+      // of fewer members in flight stay in L2 until their lines fill
+      // (65537, batch 2047: 1.44 -> 1.03 ms in f32; 2.28 -> 2.08 ms for the
+      // f64 plan). The launch asks for more than half of the CU's 160 KB LDS.
+      if (err == hipSuccess) {
+        a.in = work;
+        a.out = static_cast<char *>(out) + (size_t)(b0 * od_) * eb;
+        a.ostride = os_;
+        a.odist = od_;
+        a.tw = eout_.tw.p;
+        a.desc = eout_.d();
+        a.aux1 = perm_.p;
+        a.n = n1_;
+        a.count = nb * n2_;
+        err = cout_.e->launch(a, (a.count + cout_.e->tpb - 1) / cout_.e->tpb,
+                              std::max<size_t>(81 * 1024, cout_.e->lds_bytes(n1_)), s);
+      }
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(rader3 n=" << n_ << " N=" << n1_ * n2_ << " = " << n1_ << "x" << n2_ << " g=" << gen_
+      << " chunk=" << chunk_ << " batch=" << batch_ << " " << chain_str(cin_) << " "
+      << chain_str(cmid_) << " " << chain_str(cout_) << ")";
+    return o.str();
+  }
+
+ private:
+  int64_t n_, n1_, n2_;
+  int prec_;
+  bool bwd_;
+  int64_t batch_, is_, id_, os_, od_, chunk_;
+  Choice cin_, cmid_, cout_;
+  EngineData ein_, emid_, eout_;
+  int shift_ = 0;
+  int64_t gen_ = 0;
+  DevMem split_, B_, perm_;
+};
+
+// The most balanced split N = n1 * n2 (n1 a power of two) with compiled
+// Rader column passes on n1 and a row pass on n2.
+static bool rd3_choice(int64_t N, int prec, bool bwd, int64_t *n1_out, Choice *cin, Choice *cmid,
+                       Choice *cout) {
+  double best = 1e30;
+  for (int64_t n1 = 2; n1 < N; n1 *= 2) {
+    if (N % n1) break;
+    const int64_t n2 = N / n1;
+    Choice ci = choose(KK_RD_COL_IN, n1, prec, bwd);
+    Choice cm = choose(KK_CZ_ROW_MID, n2, prec, bwd);
+    Choice co = choose(KK_RD_COL_OUT, n1, prec, bwd);
+    if (!ci.e || !cm.e || !co.e) continue;
+    const double bal = std::fabs(std::log((double)n1 / (double)n2));
+    if (bal < best) {
+      best = bal;
+      *n1_out = n1;
+      *cin = ci;
+      *cmid = cm;
+      *cout = co;
+    }
+  }
+  return best < 1e30;
+}
+
 bool cz3_choice(int64_t n, int prec, bool bwd, Cz3Choice *out) {
   // candidate padded lengths M >= 2n-2 in increasing order: 2^k and 3 * 2^k
   int64_t P2 = 1;
@@ -466,6 +648,24 @@ std::unique_ptr<Node> build_bluestein(const Problem &p, std::string *why) {
   if (c.e)
     return finish(std::make_unique<FusedConvNode>(false, c, p.n, M, p.prec, p.howmany, p.istride,
                                                   id, p.ostride, od));
+  // Rader in three passes for a prime whose n - 1 splits into compiled
+  // column and row sides: N = n - 1 points instead of the chirp-z's M >= 2n - 2
+  if (p.n > 2 && spf(p.n) == p.n && p.n - 1 <= (int64_t(1) << 30)) {
+    int64_t n1 = 0;
+    Choice ci, cm, co;
+    if (rd3_choice(p.n - 1, p.prec, bwd, &n1, &ci, &cm, &co)) {
+      const int64_t N = p.n - 1;
+      const int64_t chunk = fourstep_chunk(N, p.prec, p.howmany);
+      auto node = std::make_unique<Rader3Node>(p.n, n1, N / n1, p.prec, bwd, p.howmany,
+                                               p.istride, id, p.ostride, od, chunk, ci, cm, co);
+      const hipError_t err = node->init();
+      if (err != hipSuccess) {
+        *why = std::string("Rader setup failed: ") + hipGetErrorString(err);
+        return nullptr;
+      }
+      return node;
+    }
+  }
   // chirp-z in three passes when compiled column and row engines cover a
   // split of a padded length 2^k or 3 * 2^k
   {

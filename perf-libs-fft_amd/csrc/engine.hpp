@@ -40,10 +40,12 @@ struct StaticEngine {
   static constexpr int LDS_PER = lds_padded<PAD>(N);
   static constexpr bool USES_LDS = NS > 1;
 
-  template <int SI, bool TO_LDS, bool LDS_IN, class Load, class Store, bool REGS_IN = false>
+  template <int SI, bool TO_LDS, bool LDS_IN, class Load, class Store, bool REGS_IN = false,
+            bool KEEP = false>
   static __device__ __forceinline__ void stage(cpx<C> *lds, const cpx<C> *__restrict__ tw,
                                                int t, bool valid, Load &load, Store &store,
-                                               const cpx<C> *vin = nullptr) {
+                                               const cpx<C> *vin = nullptr,
+                                               cpx<C> *keep = nullptr) {
     constexpr int R = Ch::radix(SI);
     constexpr int L = Ch::L(SI);
     constexpr int NB = N / R;
@@ -86,7 +88,10 @@ struct StaticEngine {
     for (int k = 0; k < KB; ++k) {
       const int b = bidx(k);
       const bool act = (NB % TPT == 0) || (b < NB);
-      if constexpr (LAST && !TO_LDS) {
+      if constexpr (LAST && KEEP) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) keep[k * R + m] = v[k][m];
+      } else if constexpr (LAST && !TO_LDS) {
         if (act && valid) {
 #pragma unroll
           for (int m = 0; m < R; ++m) store(b + m * NB, v[k][m]);
@@ -106,7 +111,9 @@ struct StaticEngine {
       }
     }
     if constexpr (!LAST || TO_LDS) __syncthreads();
-    if constexpr (!LAST) stage<SI + 1, TO_LDS, false>(lds, tw, t, valid, load, store);
+    if constexpr (!LAST)
+      stage<SI + 1, TO_LDS, false, Load, Store, false, KEEP>(lds, tw, t, valid, load, store,
+                                                             nullptr, keep);
   }
 
   // Results stay in LDS (natural order, lds[lds_pad(e)]) when TO_LDS. With
@@ -128,6 +135,17 @@ struct StaticEngine {
                                                   bool valid = true) {
     auto none = [](int) { return cpx<C>{C(0), C(0)}; };
     stage<0, TO_LDS, false, decltype(none), Store, true>(lds, tw, t, valid, none, store, vin);
+  }
+
+  // run_regs with the last stage's outputs left in registers:
+  // keep[k * RL + m] = element b + m * (N / RL) of butterfly b = t + k * TPT
+  // (RL the last radix). Other threads may still read LDS on return.
+  static __device__ __forceinline__ void run_regs_keep(const cpx<C> *vin, cpx<C> *lds,
+                                                       const cpx<C> *tw, int t, cpx<C> *keep) {
+    auto none = [](int) { return cpx<C>{C(0), C(0)}; };
+    auto drop = [](int, cpx<C>) {};
+    stage<0, false, false, decltype(none), decltype(drop), true, true>(lds, tw, t, true, none,
+                                                                       drop, vin, keep);
   }
 };
 

@@ -26,7 +26,8 @@ struct KArgs {
   const void *aux0;          // kernel-specific tables (four-step twiddles, chirps, ...)
   const void *aux1;
   const void *aux2;          // per-member records of a gathered launch, or null
-  int64_t count;             // transforms (or columns / rows) in this launch
+  void *aux3;                // writable per-member slots (Rader: x_0 and A_0), or null
+  int64_t count;            // transforms (or columns / rows) in this launch
   int64_t istride, idist;
   int64_t ostride, odist;
   int64_t n;                 // length handled by the engine

@@ -1,5 +1,6 @@
-// kernels_cz.hpp -- chirp-z (Bluestein whose DFT_M is a four-step) in three
-// HBM passes on the compile-time column / row engines.
+// kernels_cz.hpp -- chirp-z (Bluestein whose DFT_M is a four-step) and
+// Rader (prime n, N = n - 1 a four-step length) in three HBM passes on the
+// compile-time column / row engines.
 //
 // With M = n1 * n2 >= 2n - 2, c_j the chirp and B = DFT_M(b) / M
 // (bluestein.hpp:66-98), y = conj-chirp o IDFT_M(B o DFT_M(c o x)) is the
@@ -30,6 +31,18 @@
 
 namespace plf {
 
+// Logical block index that keeps consecutive indices on one XCD. The
+// dispatcher hands block b to XCD b mod 8; with this order the column tiles
+// of one member, which share the cache lines at their edges (and, for
+// Rader's permuted columns, every line of the member's input), run in one L2.
+// Block b is the (b >> 3)-th block of XCD x = b & 7, and XCD x owns the
+// q + [x < r] logical indices after those of XCDs 0..x-1 (q = G >> 3,
+// r = G & 7), so the map is a bijection of [0, G).
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
+  const int64_t x = b & 7, q = G >> 3, r = G & 7;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (b >> 3);
+}
+
 template <typename C>
 __device__ __forceinline__ cpx<C> cz_twiddle(const cpx<C> *__restrict__ split, int shift,
                                              int64_t e) {
@@ -44,7 +57,7 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_col_in(KArgs a) {
   using C = typename Eng::C;
   const int g = threadIdx.x % TPB;
   const int t = threadIdx.x / TPB;
-  const int64_t col = (int64_t)blockIdx.x * TPB + g;
+  const int64_t col = xcd_block(blockIdx.x, gridDim.x) * TPB + g;
   const bool valid = col < a.count;
   const int64_t n1 = a.n1, n2 = a.n2;
   const int64_t bi = valid ? col / n2 : 0, c2 = valid ? col % n2 : 0;
@@ -68,13 +81,24 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_col_in(KArgs a) {
       [&](int k1, cpx<C> v) { ws[(int64_t)k1 * n2] = cmul(v, cz_twiddle(split, shift, c2 * k1)); });
 }
 
+// Row pass. B is loaded with the row, at the slots where the second
+// transform's stage 0 takes its inputs, and the W_M twiddles during the
+// second transform, at the slots where its last stage leaves its outputs
+// (registers, not LDS). Between the two transforms one LDS pass turns the
+// first's natural-order output into the second's stage-0 layout, applying
+// B and the swap on the way.
 template <typename S, bool BWD, int TPB, class Eng>
 __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_row_mid(KArgs a) {
   using C = typename Eng::C;
+  using Ch = typename Eng::Ch;
   constexpr int TPT = Eng::TPT;
+  constexpr int N = Ch::N;
+  constexpr int R0 = Ch::radix(0), RL = Ch::radix(Ch::count - 1);
+  constexpr int NB0 = N / R0, NBL = N / RL;
+  constexpr int KB0 = (NB0 + TPT - 1) / TPT, KBL = (NBL + TPT - 1) / TPT;
   const int g = threadIdx.x / TPT;
   const int t = threadIdx.x % TPT;
-  const int64_t row = (int64_t)blockIdx.x * TPB + g;
+  const int64_t row = xcd_block(blockIdx.x, gridDim.x) * TPB + g;
   const bool valid = row < a.count;
   const int64_t n1 = a.n1, n2 = a.n2;
   const int64_t bi = valid ? row / n1 : 0, k1 = valid ? row % n1 : 0;
@@ -85,17 +109,51 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_row_mid(KArgs a) {
   const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw);
   const int shift = a.shift;
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  cpx<C> bv[KB0 * R0], wv[KBL * RL];
+#pragma unroll
+  for (int k = 0; k < KB0; ++k)
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const int b = t + k * TPT;
+      const bool act = (NB0 % TPT == 0) || (b < NB0);
+      bv[k * R0 + m] = (act && valid) ? Brow[b + m * NB0] : cpx<C>{C(0), C(0)};
+    }
   Eng::template run<true>(
       a.desc, lds, tw, t, valid, [&](int e) { return rowp[e]; }, [&](int, cpx<C>) {});
-  if (valid)
-    for (int k2 = t; k2 < n2; k2 += TPT) {
-      const int p = lds_pad<Eng::PAD>(k2);
-      lds[p] = cswap(cmul(lds[p], Brow[k2]));
+#pragma unroll
+  for (int k = 0; k < KBL; ++k)
+#pragma unroll
+    for (int m = 0; m < RL; ++m) {
+      const int b = t + k * TPT;
+      const bool act = (NBL % TPT == 0) || (b < NBL);
+      wv[k * RL + m] = (act && valid) ? cz_twiddle(split, shift, k1 * (b + m * NBL))
+                                      : cpx<C>{C(0), C(0)};
     }
-  __syncthreads();
-  Eng::template run<false, true>(
-      a.desc, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; },
-      [&](int m2, cpx<C> v) { rowp[m2] = cmul(v, cz_twiddle(split, shift, k1 * m2)); });
+  // Rader (aux3 set): bin 0 of the forward transform, A_0 = sum of the
+  // permuted inputs, goes to the member's slot for X_0 = x_0 + A_0
+  cpx<C> *slot = static_cast<cpx<C> *>(a.aux3);
+  cpx<C> vin[KB0 * R0];
+#pragma unroll
+  for (int k = 0; k < KB0; ++k)
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      const int b = t + k * TPT;
+      const bool act = (NB0 % TPT == 0) || (b < NB0);
+      const cpx<C> A = act ? lds[lds_pad<Eng::PAD>(b + m * NB0)] : cpx<C>{C(0), C(0)};
+      if (k == 0 && m == 0 && slot && valid && k1 == 0 && t == 0) slot[2 * bi + 1] = A;
+      vin[k * R0 + m] = cswap(cmul(A, bv[k * R0 + m]));
+    }
+  __syncthreads();  // the second transform's first exchange rewrites LDS
+  cpx<C> out[KBL * RL];
+  Eng::E::run_regs_keep(vin, lds, tw, t, out);
+  if (valid)
+#pragma unroll
+    for (int k = 0; k < KBL; ++k)
+#pragma unroll
+      for (int m = 0; m < RL; ++m) {
+        const int b = t + k * TPT;
+        if ((NBL % TPT == 0) || (b < NBL)) rowp[b + m * NBL] = cmul(out[k * RL + m], wv[k * RL + m]);
+      }
 }
 
 template <typename S, bool BWD, int TPB, class Eng>
@@ -105,7 +163,7 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_col_out(KArgs a) {
   using C = typename Eng::C;
   const int g = threadIdx.x % TPB;
   const int t = threadIdx.x / TPB;
-  const int64_t col = (int64_t)blockIdx.x * TPB + g;
+  const int64_t col = xcd_block(blockIdx.x, gridDim.x) * TPB + g;
   const bool valid = col < a.count;
   const int64_t n1 = a.n1, n2 = a.n2;
   const int64_t bi = valid ? col / n2 : 0, c2 = valid ? col % n2 : 0;
@@ -124,6 +182,78 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_cz_col_out(KArgs a) {
         if (m < nn) {
           const cpx<C> y = cmul(cswap(v), chirp[m]);
           out[m * os] = ST::st(BWD ? cswap(y) : y);
+        }
+      });
+}
+
+// Rader in three passes for a prime n with N = n - 1 = n1 * n2 (the
+// reference's rader.hpp:51-90, with its convolution on the four-step
+// passes above). g generates the nonzero residues mod n, perm[p] = g^p mod n,
+// a_p = x[perm[p]] and b_p = W_n^{perm[(N - p) mod N]}; with B = DFT_N(b) / N,
+// X[perm[(N - q) mod N]] = x_0 + (a (*) b)_q and X_0 = x_0 + sum_p a_p.
+// The row pass is k_cz_row_mid with aux3 set (it keeps A_0 = sum_p a_p).
+// KArgs as for the chirp-z passes, except: aux1 = perm (int32, N entries)
+// in the column passes, aux3 = two slots per member of the launch (x_0,
+// A_0; compute precision) and bigdist unused (no padding).
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_rd_col_in(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  const int g = threadIdx.x % TPB;
+  const int t = threadIdx.x / TPB;
+  const int64_t col = xcd_block(blockIdx.x, gridDim.x) * TPB + g;
+  const bool valid = col < a.count;
+  const int64_t n1 = a.n1, n2 = a.n2;
+  const int64_t bi = valid ? col / n2 : 0, c2 = valid ? col % n2 : 0;
+  const E *in = static_cast<const E *>(a.in) + bi * a.idist;
+  cpx<C> *ws = static_cast<cpx<C> *>(a.out) + bi * n1 * n2 + c2;
+  const cpx<C> *split = static_cast<const cpx<C> *>(a.aux0);
+  const int *perm = static_cast<const int *>(a.aux1) + c2;
+  const int64_t is = a.istride;
+  const int shift = a.shift;
+  if (valid && c2 == 0 && t == 0) {
+    const cpx<C> x0 = ST::ld(in[0]);
+    static_cast<cpx<C> *>(a.aux3)[2 * bi] = BWD ? cswap(x0) : x0;
+  }
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) {
+        const cpx<C> v = ST::ld(in[(int64_t)perm[n2 * e] * is]);
+        return BWD ? cswap(v) : v;
+      },
+      [&](int k1, cpx<C> v) { ws[(int64_t)k1 * n2] = cmul(v, cz_twiddle(split, shift, c2 * k1)); });
+}
+
+template <typename S, bool BWD, int TPB, class Eng>
+__global__ void __launch_bounds__(Eng::TPT *TPB) k_rd_col_out(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename Eng::C;
+  const int g = threadIdx.x % TPB;
+  const int t = threadIdx.x / TPB;
+  const int64_t col = xcd_block(blockIdx.x, gridDim.x) * TPB + g;
+  const bool valid = col < a.count;
+  const int64_t n1 = a.n1, n2 = a.n2, N = n1 * n2;
+  const int64_t bi = valid ? col / n2 : 0, c2 = valid ? col % n2 : 0;
+  const cpx<C> *ws = static_cast<const cpx<C> *>(a.in) + bi * N + c2;
+  E *out = static_cast<E *>(a.out) + bi * a.odist;
+  const int *perm = static_cast<const int *>(a.aux1);
+  const cpx<C> *slot = static_cast<const cpx<C> *>(a.aux3) + 2 * bi;
+  const cpx<C> x0 = slot[0];
+  const int64_t os = a.ostride;
+  cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(a.n);
+  Eng::template run<false>(
+      a.desc, lds, static_cast<const cpx<C> *>(a.tw), t, valid,
+      [&](int e) { return ws[(int64_t)e * n2]; },
+      [&](int m1, cpx<C> v) {
+        const int64_t q = c2 + n2 * m1;
+        const cpx<C> r = cadd(x0, cswap(v));
+        out[(int64_t)perm[q == 0 ? 0 : N - q] * os] = ST::st(BWD ? cswap(r) : r);
+        if (q == 0) {
+          const cpx<C> y0 = cadd(x0, slot[1]);
+          out[0] = ST::st(BWD ? cswap(y0) : y0);
         }
       });
 }

@@ -1,4 +1,4 @@
-// kernels_f32_cz.hip -- three-pass chirp-z kernels with FP32 arithmetic
+// kernels_f32_cz.hip -- three-pass chirp-z and Rader kernels with FP32 arithmetic
 // (f32, f16 and bf16 storage; the column and row engines of the four-step
 // passes, sides 64..1024).
 #include "kernels_cz.hpp"
@@ -24,6 +24,17 @@ void register_cz_f32_compute(std::vector<KEntry> &r) {
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   16,  32, 16, 16);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   32,  16, 16, 16, 2);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   64,  16, 16, 16, 4);
+  // Rader's permuted column passes, the same column sides
+  PLF_STATIC(k_rd_col_in, KK_RD_COL_IN, S, P,    8,  16, 8, 8);
+  PLF_STATIC(k_rd_col_in, KK_RD_COL_IN, S, P,    8,  16, 16, 8);
+  PLF_STATIC(k_rd_col_in, KK_RD_COL_IN, S, P,   16,  32, 16, 16);
+  PLF_STATIC(k_rd_col_in, KK_RD_COL_IN, S, P,   32,  16, 16, 16, 2);
+  PLF_STATIC(k_rd_col_in, KK_RD_COL_IN, S, P,   64,  16, 16, 16, 4);
+  PLF_STATIC(k_rd_col_out, KK_RD_COL_OUT, S, P,    8,  16, 8, 8);
+  PLF_STATIC(k_rd_col_out, KK_RD_COL_OUT, S, P,    8,  16, 16, 8);
+  PLF_STATIC(k_rd_col_out, KK_RD_COL_OUT, S, P,   16,  32, 16, 16);
+  PLF_STATIC(k_rd_col_out, KK_RD_COL_OUT, S, P,   32,  16, 16, 16, 2);
+  PLF_STATIC(k_rd_col_out, KK_RD_COL_OUT, S, P,   64,  16, 16, 16, 4);
   // row sides with a factor 3, for M = 3 * 2^k
   PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,    8,  16, 8, 4, 3);
   PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   16,  16, 16, 4, 3);

@@ -42,14 +42,16 @@ enum KKind : int {
   KK_CZ_COL_IN = 18,   // chirp-z in three passes: chirp-in column pass
   KK_CZ_ROW_MID = 19,  //   row pass: DFT, x B, first inverse DFT
   KK_CZ_COL_OUT = 20,  //   chirp-out column pass
+  KK_RD_COL_IN = 21,   // Rader in three passes: permuted-input column pass
+  KK_RD_COL_OUT = 22,  //   permuted-output column pass (rows: KK_CZ_ROW_MID)
 };
 inline const char *kkind_name(int k) {
   static const char *names[] = {"c2c",       "col",       "row",       "r2c_half", "c2r_half",
                                 "r2c_full",  "c2r_full",  "bluestein", "rader",    "dct2_half",
                                 "dct2_full", "dct3_half", "dct3_full", "c2c_unit", "col_r2c",
                                 "row_r2c",   "col_c2r",   "row_c2r",  "cz_col_in", "cz_row_mid",
-                                "cz_col_out"};
-  return (k >= 0 && k <= 20) ? names[k] : "?";
+                                "cz_col_out", "rd_col_in", "rd_col_out"};
+  return (k >= 0 && k <= 22) ? names[k] : "?";
 }
 
 using KLaunch = hipError_t (*)(const KArgs &, int64_t grid, size_t lds, hipStream_t);

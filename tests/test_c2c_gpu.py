@@ -256,7 +256,9 @@ def test_c2c_prime_plan_choice():
     assert "bluestein" in P.Plan("c2c", 59, -1, "f32").describe()
 
 
-@pytest.mark.parametrize("n,prec", [(65537, "f32"), (65537, "f64"), (40009, "f32")])
+# primes past one LDS tile whose n - 1 has no compiled four-step split
+# (65520 = 2^4 3^2 5 7 13) and a composite with a large prime factor
+@pytest.mark.parametrize("n,prec", [(65521, "f32"), (65521, "f64"), (40009, "f32")])
 def test_c2c_large_prime_chirp_z(torch_cuda, orc, n, prec):
     torch = torch_cuda
     hm = 3
@@ -264,9 +266,65 @@ def test_c2c_large_prime_chirp_z(torch_cuda, orc, n, prec):
     xd = to_dev(torch, x)
     yd = torch.empty_like(xd)
     plan = P.Plan("c2c", n, -1, prec, howmany=hm)
+    assert "chirpz" in plan.describe()
     plan.execute(xd, yd)
     ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
     assert relerr(from_dev(torch, yd), ref) < TOL[prec] * 2, plan.describe()
+
+
+# Rader in three passes: primes whose n - 1 splits into compiled column and
+# row sides (65537: 256 x 256; 786433: 1024 x 768 in FP32).
+# (f16 / bf16 storage: tests/test_half_gpu.py, n = 65537)
+@pytest.mark.parametrize("n,prec", [(65537, "f32"), (65537, "f64"), (786433, "f32")])
+def test_c2c_rader_three_pass(torch_cuda, orc, n, prec):
+    torch = torch_cuda
+    hm = 3 if n < 100000 else 1
+    x = rng.complex_normal(n + 7, n * hm).astype(CPLX_NP[prec])
+    xd = to_dev(torch, x)
+    for direction in (-1, 1):
+        plan = P.Plan("c2c", n, direction, prec, howmany=hm)
+        print(f"rader3 {prec} n={n} dir={direction}: {plan.describe()}", flush=True)
+        assert "rader3" in plan.describe()
+        yd = torch.empty_like(xd)
+        plan.execute(xd, yd)
+        ref = orc.c2c_batched(x.astype(np.complex128), n, direction, hm, 1, n, 1, n, n * hm)
+        assert relerr(from_dev(torch, yd), ref) < TOL[prec] * 2, plan.describe()
+
+
+def test_c2c_rader_three_pass_layouts(torch_cuda, orc):
+    """In place, and strided input / output (istride 2, idist 2n + 64), of
+    the three-pass Rader plan; plus delta and tone known pairs."""
+    torch = torch_cuda
+    n, hm = 65537, 2
+    x = rng.complex_normal(n + 11, n * hm).astype(np.complex64)
+    ref = orc.c2c_batched(x.astype(np.complex128), n, -1, hm, 1, n, 1, n, n * hm)
+    xd = to_dev(torch, x)
+    plan = P.Plan("c2c", n, -1, "f32", howmany=hm)
+    assert "rader3" in plan.describe()
+    plan.execute(xd, xd)
+    assert relerr(from_dev(torch, xd), ref) < TOL["f32"] * 2
+    idist = 2 * n + 64
+    xs = np.zeros(idist * hm, np.complex64)
+    for b in range(hm):
+        xs[b * idist:b * idist + 2 * n:2] = x[b * n:(b + 1) * n]
+    xsd = to_dev(torch, xs)
+    ysd = torch.zeros_like(xsd)
+    plan = P.Plan("c2c", n, -1, "f32", howmany=hm, istride=2, idist=idist, ostride=2,
+                  odist=idist)
+    plan.execute(xsd, ysd)
+    ys = from_dev(torch, ysd)
+    got = np.concatenate([ys[b * idist:b * idist + 2 * n:2] for b in range(hm)])
+    assert relerr(got, ref) < TOL["f32"] * 2
+    k0 = n // 3
+    xp = np.zeros((2, n), np.complex64)
+    xp[0, 0] = 1
+    xp[1, k0] = 1
+    xd = to_dev(torch, xp)
+    yd = torch.empty_like(xd)
+    P.Plan("c2c", n, -1, "f32", howmany=2).execute(xd, yd)
+    y = from_dev(torch, yd)
+    assert np.max(np.abs(y[0] - 1)) < 1e-5
+    assert np.max(np.abs(y[1] - np.exp(-2j * np.pi * np.arange(n) * k0 / n))) < 2e-5
 
 
 @pytest.mark.parametrize("n", [97, 509, 2053, 8191])
@@ -302,11 +360,11 @@ def test_size_limits_fail_cleanly(torch_cuda):
     assert ei.value.status == P.PLFFT_PLAN_CREATION_FAILED
 
 
-@pytest.mark.parametrize("n", [65536, 65537])
+@pytest.mark.parametrize("n", [65536, 65537, 40009])
 def test_one_plan_from_four_host_threads(torch_cuda, orc, n):
     """include/plfft.h allows concurrent execute of one plan from several
-    threads; multi-pass plans (four-step 65536, chirp-z 65537) take one
-    workspace per concurrent synchronous call from the plan's pool."""
+    threads; multi-pass plans (four-step 65536, Rader 65537, chirp-z 40009)
+    take one workspace per concurrent synchronous call from the plan's pool."""
     import threading
     torch = torch_cuda
     hm = 3

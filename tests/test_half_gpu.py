@@ -41,7 +41,8 @@ def _run(torch, plan, xs, out_reals):
 
 
 @pytest.mark.parametrize("prec", ["f16", "bf16"])
-@pytest.mark.parametrize("n", [16, 32, 256, 1000, 4096, 15015, 65536, 97, 1009, 2053])
+@pytest.mark.parametrize("n", [16, 32, 256, 1000, 4096, 15015, 65536, 97, 1009, 2053, 40009,
+                               65537])
 def test_c2c_half(torch_cuda, orc, prec, n):
     torch = torch_cuda
     hm = max(1, min(8, (1 << 16) // n)) + 1

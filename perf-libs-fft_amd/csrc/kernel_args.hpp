@@ -47,4 +47,14 @@ struct CzMember {
   int64_t chirp_off, B_off;
 };
 
+// One transform of a gathered one-block Bluestein launch (kernels_conv.hpp):
+// n points read at in_off with unit stride and written at out_off with
+// stride ostride (elements of the kernel's storage type), with the chirp
+// and B tables at chirp_off and B_off in aux1 and aux0.
+struct BlueMember {
+  int64_t n;
+  int64_t in_off, out_off, ostride;
+  int64_t chirp_off, B_off;
+};
+
 }  // namespace plf

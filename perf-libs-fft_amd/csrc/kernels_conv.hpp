@@ -20,6 +20,9 @@
 namespace plf {
 
 // aux0: B (M entries), aux1: chirp c (n entries); a.n = M, a.n1 = n.
+// Gathered launch (aux2 = BlueMember records, one per transform; a mixed
+// batch's members and rows of equal M): length, offsets, output stride and
+// table offsets per record, unit input stride.
 template <typename S, bool BWD, int TPB, class Eng>
 __global__ void __launch_bounds__(Eng::TPT *TPB) k_bluestein(KArgs a) {
   using ST = StorageT<S>;
@@ -31,14 +34,16 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_bluestein(KArgs a) {
   const int64_t tr = (int64_t)blockIdx.x * TPB + g;
   const bool valid = tr < a.count;
   const int M = (int)a.n;
-  const int n = (int)a.n1;
-  const E *in = static_cast<const E *>(a.in) + (valid ? tr * a.idist : 0);
-  E *out = static_cast<E *>(a.out) + (valid ? tr * a.odist : 0);
+  const BlueMember *mt = static_cast<const BlueMember *>(a.aux2);
+  const BlueMember *rec = mt ? mt + (valid ? tr : 0) : nullptr;
+  const int n = mt ? (int)rec->n : (int)a.n1;
+  const E *in = static_cast<const E *>(a.in) + (mt ? rec->in_off : valid ? tr * a.idist : 0);
+  E *out = static_cast<E *>(a.out) + (mt ? rec->out_off : valid ? tr * a.odist : 0);
   cpx<C> *lds = dyn_lds<C>() + g * Eng::lds_per(M);
   const cpx<C> *tw = static_cast<const cpx<C> *>(a.tw);
-  const cpx<C> *B = static_cast<const cpx<C> *>(a.aux0);
-  const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1);
-  const int64_t is = a.istride, os = a.ostride;
+  const cpx<C> *B = static_cast<const cpx<C> *>(a.aux0) + (mt ? rec->B_off : 0);
+  const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1) + (mt ? rec->chirp_off : 0);
+  const int64_t is = mt ? 1 : a.istride, os = mt ? rec->ostride : a.ostride;
   Eng::template run<true>(
       a.desc, lds, tw, t, valid,
       [&](int e) {

@@ -3,9 +3,12 @@
 // Members are split into tasks of one 256-thread block each:
 //   ONE       a whole member in LDS: direct (runtime-chain engine) or
 //             Bluestein (chirp, DFT_M, pointwise B, DFT_M, chirp)
-//   COL/ROW   four-step tiles: 16 (or 8) columns per pass through LDS,
+//   COL/ROW   four-step tiles: 16 (or 8; 64 or 256 for short columns)
+//             columns per pass through LDS,
 //             twiddled into the member's workspace; then 16 (or 8) rows
 //             with a transposed store
+// (the rows of a four-step member whose n2 has a large prime factor run on
+// the compiled Bluestein kernel, kernels_conv.hpp, after these columns)
 // (chirp-z members run on the three-pass kernels of kernels_cz.hpp, in
 // groups). A launch runs one dependency level (ONE + COL, then ROW); stream
 // order separates the levels, so blocks never wait for each other and need
@@ -28,6 +31,8 @@ struct MixedExec {
   using EngD = DynamicEngine<C, 256, 16, true, true>;   // one member per block
   using Eng16 = DynamicEngine<C, 16, 16, true, true>;   // 16 lanes x 16 threads
   using Eng8 = DynamicEngine<C, 32, 16, true, true>;    // 8 lanes x 32 threads
+  using Eng64 = DynamicEngine<C, 4, 16, true, true>;    // 64 lanes x 4 threads
+  using Eng256 = DynamicEngine<C, 1, 16, true, true>;   // 256 lanes x 1 thread
   static constexpr int PAD = 5;
 
   // backward = swap o forward o swap (swap exchanges re and im)
@@ -98,7 +103,10 @@ struct MixedExec {
     EngT::template run<false>(
         d, lds, tw, t, valid, [&](int e) { return io(a, ST::ld(in[col + (int64_t)e * n2])); },
         [&](int k1, cpx<C> v) {
-          ws[(int64_t)k1 * n2 + col] = cmul(v, split_tw(aux, x, (int64_t)col * k1));
+          // kind 4: the rows run on the Bluestein kernel, which swaps its
+          // input again for a backward transform
+          const cpx<C> w = cmul(v, split_tw(aux, x, (int64_t)col * k1));
+          ws[(int64_t)k1 * n2 + col] = x.kind == 4 ? io(a, w) : w;
         });
   }
 
@@ -145,7 +153,9 @@ struct MixedExec {
           one_block(a, x, lds);
         } else {
           if (x.lanes0 == 16) col_tile<Eng16, 16>(a, tk.first, tk.end, x, lds);
-          else col_tile<Eng8, 8>(a, tk.first, tk.end, x, lds);
+          else if (x.lanes0 == 8) col_tile<Eng8, 8>(a, tk.first, tk.end, x, lds);
+          else if (x.lanes0 == 64) col_tile<Eng64, 64>(a, tk.first, tk.end, x, lds);
+          else col_tile<Eng256, 256>(a, tk.first, tk.end, x, lds);
         }
       } else {  // MT_ROW
         if (x.lanes1 == 16) row_tile<Eng16, 16>(a, tk.first, x, lds);

@@ -12,16 +12,20 @@ namespace plf {
 // Per-member description (device memory).
 struct MixedXf {
   int n;         // transform length
-  int kind;      // 0 direct, 1 four-step, 2 Bluestein in one block, 3 chirp-z (grouped, no tiles)
-  int len;       // engine length of a one-block member (n or M); M for chirp-z
-  int n1, n2;    // four-step split of n (kind 1)
+  int kind;      // 0 direct, 1 four-step, 2 Bluestein in one block, 3 chirp-z (grouped,
+                 // no tiles), 4 four-step whose rows are Bluestein in one block
+  int len;       // engine length of a one-block member (n or M); M for chirp-z; kind 4:
+                 // the rows' padded length
+  int n1, n2;    // four-step split of n (kinds 1, 4)
   int d0, d1;    // engine descriptors: one-block (d0) or column (d0) / row (d1)
   int shift;     // W_N split: N = n1*n2, W^e = hi[e >> shift] * lo[e & mask]
-  int lanes0, lanes1;  // tile width of the column / row passes (16 or 8)
+  int lanes0, lanes1;  // tile width of the column / row passes (16 or 8; columns of
+                       // Bluestein-row members also 64 or 256)
   int64_t in_off, out_off;  // complex elements
   int64_t ws_off;           // complex elements of workspace (kind 1: n)
   int64_t tw4_off;          // split twiddles in aux: lo (2^shift), then hi
   int64_t chirp_off;        // chirp c_k (n entries) in aux, at the residue of in_off mod 16
+                            // (kind 4: the rows' chirp, n2 entries)
   int64_t B_off;            // B = DFT_M(b)/M (M entries) in aux; chirp-z: row-major n1 x n2
 };
 

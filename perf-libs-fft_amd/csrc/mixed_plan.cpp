@@ -1,14 +1,19 @@
-// mixed_plan.cpp -- a batch of mixed lengths in at most three launches.
+// mixed_plan.cpp -- a batch of mixed lengths in a fixed set of launches.
 //
 // The reference executes a batch as one plan per length (src/plfft.cpp:
 // 283-330 builds a plan for a single n; a caller with many lengths loops over
-// plans). Here the whole batch runs as at most three launches, one per
-// dependency level of the multi-pass schedules (kernels_mixed.hpp).
-// Each member gets one of four schedules:
+// plans). Here the whole batch runs as two tile launches, one per dependency
+// level of the multi-pass schedules (kernels_mixed.hpp), plus grouped
+// launches of the compiled Bluestein and chirp-z kernels.
+// Each member gets one of five schedules:
 //   direct      smooth n whose radix chain fits one block (n <= 4096)
 //   four-step   smooth n above that: column tiles, then row tiles, through
 //               a workspace region of its own
-//   Bluestein   other n with a padded length M <= 4096: one block
+//   Bluestein   other n with a padded length M <= 4096: one block (on the
+//               compiled fused kernel, grouped by M, for FP32 / FP64)
+//   CT-Bluestein  n = n1 * n2 with n1 smooth and n2 a one-block Bluestein
+//               length: column tiles, then the rows as compiled Bluestein
+//               transforms grouped by M (FP32 / FP64)
 //   chirp-z     other n: Bluestein around two four-step transforms of M,
 //               fused into three passes
 // (the same algorithms as the single-length planner: Stockham passes,
@@ -73,6 +78,15 @@ int lanes_for(int64_t len) {
   return 0;
 }
 
+// Tile width of the column pass of a Bluestein-row member, whose column
+// side is often short: 256 columns of one thread each, or 64 of four, keep
+// every task a full block of work.
+int lanes_wide(int64_t len) {
+  if (engine_fits(len, 1)) return 256;
+  if (engine_fits(len, 4)) return 64;
+  return lanes_for(len);
+}
+
 struct Split {
   int64_t n1 = 0, n2 = 0;
   int l0 = 0, l1 = 0;
@@ -116,10 +130,32 @@ bool best_split(int64_t N, Split *out) {
 
 struct Schedule {
   int kind = -1;  // 0 direct, 1 four-step, 2 Bluestein one block,
-                  // 3 chirp-z on the three-pass compiled kernels
-  int64_t len = 0;  // one-block engine length (n or M); M for chirp-z
+                  // 3 chirp-z on the three-pass compiled kernels,
+                  // 4 four-step n = n1 * n2 with Bluestein rows
+  int64_t len = 0;  // one-block engine length (n or M); M for chirp-z; kind 4: M of the rows
+  bool compiled = false;  // kind 2: on the compiled fused kernel (a Bluestein group)
   Split sp;
 };
+
+// The padded length of a one-block Bluestein DFT of n points on the
+// compiled fused kernel (FP32 / FP64 storage): the smallest power of two
+// M >= max(64, 2n - 2) up to 4096 with a compiled chain, or 0.
+int64_t blue_len_compiled(int64_t n, int prec, bool bwd) {
+  if (prec != P_F32 && prec != P_F64) return 0;
+  int64_t M = 64;
+  while (M < 2 * n - 2) M *= 2;
+  if (M > kOneBlock) return 0;
+  const Choice c = choose(KK_BLUE, M, prec, bwd);
+  return (c.e && c.e->n) ? M : 0;
+}
+
+// The padded length of a one-block Bluestein DFT of n points: the smallest
+// 7-smooth M >= 2n - 2 whose chain fits one 256-thread block, or 0.
+int64_t blue_len(int64_t n) {
+  for (int64_t M = std::max<int64_t>(2 * n - 2, 1); M <= kOneBlock; ++M)
+    if (smooth7(M) && engine_fits(M, 256)) return M;
+  return 0;
+}
 
 bool schedule_for(int64_t n, int prec, bool bwd, Schedule *s, std::string *why) {
   if (n <= kOneBlock && smooth(n) && engine_fits(n, 256)) {
@@ -132,12 +168,41 @@ bool schedule_for(int64_t n, int prec, bool bwd, Schedule *s, std::string *why) 
     s->len = n;
     return true;
   }
-  for (int64_t M = 2 * n - 2; M <= kOneBlock; ++M)
-    if (smooth7(M) && engine_fits(M, 256)) {
-      s->kind = 2;
+  if (const int64_t M = blue_len_compiled(n, prec, bwd)) {
+    s->kind = 2;
+    s->len = M;
+    s->compiled = true;
+    return true;
+  }
+  if (const int64_t M = blue_len(n)) {
+    s->kind = 2;
+    s->len = M;
+    return true;
+  }
+  // Cooley-Tukey n = n1 * n2 with a smooth column side n1 and Bluestein rows
+  // of n2 points on the compiled fused kernel (the reference's
+  // cooley_tukey_bluestein strategies, src/planner/planner.cpp:171-176,
+  // bluestein.hpp:781-941): four HBM element moves per point instead of the
+  // chirp-z's padded passes. The split with the least padded row work
+  // n1 * M log2 M (M = the rows' padded length) wins.
+  {
+    double best = 0.0;
+    for (int64_t n1 = 2; n1 <= kMaxSide && n1 < n; ++n1) {
+      if (n % n1 || !smooth7(n1)) continue;
+      const int64_t n2 = n / n1;
+      const int l0 = lanes_wide(n1);
+      if (!l0) continue;
+      const int64_t M = blue_len_compiled(n2, prec, bwd);
+      if (!M) continue;
+      const double work = (double)n1 * (double)M * std::log2((double)M);
+      if (s->kind == 4 && work >= best) continue;
+      best = work;
+      s->kind = 4;
       s->len = M;
-      return true;
+      s->sp = Split{n1, n2, l0, 0};
     }
+    if (s->kind == 4) return true;
+  }
   // chirp-z on the three-pass compiled kernels (padded length 2^k or 3 2^k)
   Cz3Choice cz;
   if (cz3_choice(n, prec, bwd, &cz)) {
@@ -253,15 +318,18 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     }
     return split_of[N] = {off, shift};
   };
-  // Bluestein members grouped by M, for batched FP64 DFTs of b. Chirp
-  // tables are placed per member below (aligned like the member's data).
+  // Bluestein lengths grouped by M, for batched FP64 DFTs of b: members of
+  // kinds 2 and 3, and the rows (n2) of kind-4 members. Chirp tables are
+  // placed per member below (aligned like the member's data).
   std::map<int64_t, std::vector<double>> chirp_host;
   std::map<int64_t, std::vector<int64_t>> blue_by_M;
-  for (const auto &kv : sched)
-    if (kv.second.kind >= 2) {
-      chirp_host[kv.first] = chirp_table(kv.first);
-      blue_by_M[kv.second.len].push_back(kv.first);
-    }
+  for (const auto &kv : sched) {
+    if (kv.second.kind < 2) continue;
+    const int64_t key = kv.second.kind == 4 ? kv.second.sp.n2 : kv.first;
+    if (chirp_host.count(key)) continue;
+    chirp_host[key] = chirp_table(key);
+    blue_by_M[kv.second.len].push_back(key);
+  }
   // pad aux (complex entries) up to an offset congruent to r mod 16
   auto align_aux = [&](int64_t r) {
     while ((int64_t)(aux.size() / 2) % 16 != r) {
@@ -294,11 +362,12 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
       return nullptr;
     }
     for (size_t j = 0; j < ns.size(); ++j) {
-      const Schedule &sj = sched[ns[j]];
+      const auto sit = sched.find(ns[j]);
       const double *src = b.data() + 2 * (size_t)M * j;
       align_aux(0);
       B_of[ns[j]] = (int64_t)(aux.size() / 2);
-      if (sj.kind >= 3) {
+      if (sit != sched.end() && sit->second.kind == 3) {
+        const Schedule &sj = sit->second;
         // chirp-z: row k1 of the middle pass reads B[k1 + n1 k2], k2 < n2,
         // so B is stored row-major by k1
         const int64_t n1 = sj.sp.n1, n2 = sj.sp.n2;
@@ -334,8 +403,9 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
   std::vector<HostTask> level[2];
   int64_t ws = 0, ipos = 0, opos = 0;
   size_t lds = 0;
-  int nkind[4] = {0, 0, 0, 0};
+  int nkind[5] = {0, 0, 0, 0, 0};
   std::map<int64_t, std::vector<CzMember>> cz_members;  // by M
+  std::map<int64_t, std::vector<BlueMember>> blue_members, blue_rows;  // by M
   for (int64_t i = 0; i < count; ++i) {
     const int64_t nn = n[i];
     const Schedule &s = sched[nn];
@@ -354,10 +424,15 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     // the member's line offset: workspace rows, chirp tables and column
     // tiles are placed at the same residue mod 16 as its input
     const int64_t r16 = ((x.in_off % 16) + 16) % 16;
-    x.chirp_off = s.kind >= 2 ? get_chirp(nn, r16) : 0;
-    x.B_off = B_of.count(nn) ? B_of[nn] : 0;
+    x.chirp_off = s.kind == 4 ? get_chirp(s.sp.n2, 0) : s.kind >= 2 ? get_chirp(nn, r16) : 0;
+    const int64_t bkey = s.kind == 4 ? s.sp.n2 : nn;
+    x.B_off = B_of.count(bkey) ? B_of[bkey] : 0;
     ++nkind[s.kind];
     const int xi = (int)i;
+    if (s.kind == 2 && s.compiled) {
+      blue_members[s.len].push_back(BlueMember{nn, x.in_off, x.out_off, 1, x.chirp_off, x.B_off});
+      continue;
+    }
     if (s.kind == 0 || s.kind == 2) {
       x.d0 = get_desc(s.len);
       lds = std::max(lds, (size_t)padded(s.len) * csize);
@@ -375,17 +450,18 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     x.lanes0 = sp.l0;
     x.lanes1 = sp.l1;
     x.d0 = get_desc(sp.n1);
-    x.d1 = get_desc(sp.n2);
-    const auto spl = get_split(s.len);
+    x.d1 = s.kind == 4 ? 0 : get_desc(sp.n2);
+    const auto spl = get_split(nn);
     x.tw4_off = spl.first;
     x.shift = spl.second;
     ws += ((r16 - ws % 16) + 16) % 16;
     x.ws_off = ws;
-    ws += s.len;
+    ws += nn;
     lds = std::max(lds, (size_t)sp.l0 * padded(sp.n1) * csize);
-    lds = std::max(lds, (size_t)sp.l1 * padded(sp.n2) * csize);
+    if (s.kind != 4) lds = std::max(lds, (size_t)sp.l1 * padded(sp.n2) * csize);
     const double ccost = sp.l0 * (double)sp.n1 * (2.0 + 0.1 * std::log2((double)sp.n1));
-    const double rcost = sp.l1 * (double)sp.n2 * (2.0 + 0.1 * std::log2((double)sp.n2));
+    const double rcost = s.kind == 4 ? 0.0
+                                     : sp.l1 * (double)sp.n2 * (2.0 + 0.1 * std::log2((double)sp.n2));
     // column tiles start where in_off + column is a multiple of the tile
     // width (a short head tile first)
     const int64_t head = (sp.l0 - r16 % sp.l0) % sp.l0;
@@ -393,6 +469,13 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
       const int64_t e = std::min<int64_t>(sp.n2, c < head ? head : c + sp.l0);
       level[0].push_back({MixedTask{MT_COL, xi, (int)c, (int)e}, ccost});
       c = e;
+    }
+    if (s.kind == 4) {
+      // rows: workspace row k1 -> bins k1 + n1 k2 (output stride n1)
+      for (int64_t k1 = 0; k1 < sp.n1; ++k1)
+        blue_rows[s.len].push_back(BlueMember{sp.n2, x.ws_off + k1 * sp.n2, x.out_off + k1, sp.n1,
+                                              x.chirp_off, x.B_off});
+      continue;
     }
     for (int64_t r = 0; r < sp.n1; r += sp.l1)
       level[1].push_back({MixedTask{MT_ROW, xi, (int)r, (int)sp.n1}, rcost});
@@ -428,6 +511,26 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
     }
     P.cz_groups_.push_back(std::move(g));
   }
+
+  // one-block Bluestein groups: members from the input, then rows from the
+  // workspace, one launch per padded length each
+  for (int src = 0; src < 2; ++src)
+    for (auto &kv : src ? blue_rows : blue_members) {
+      auto g = std::make_unique<BlueGroup>();
+      g->M = kv.first;
+      g->count = (int64_t)kv.second.size();
+      g->from_ws = src == 1;
+      g->c = choose(KK_BLUE, g->M, prec, dir > 0);
+      hipError_t err = g->c.e ? g->eng.init(g->c, g->M) : hipErrorInvalidValue;
+      if (err == hipSuccess)
+        err = g->members.upload(kv.second.data(), kv.second.size() * sizeof(BlueMember));
+      if (err != hipSuccess) {
+        *why = std::string("mixed batch: Bluestein group setup failed: ") +
+               hipGetErrorString(err);
+        return nullptr;
+      }
+      P.blue_groups_.push_back(std::move(g));
+    }
 
   std::vector<MixedTask> tasks;
   for (int L = 0; L < 2; ++L) {
@@ -467,7 +570,8 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
 
   std::ostringstream o;
   o << "(mixed count=" << count << " direct=" << nkind[0] << " fourstep=" << nkind[1]
-    << " bluestein=" << nkind[2] << " chirpz=" << nkind[3] << " (" << P.cz_groups_.size()
+    << " bluestein=" << nkind[2] << " ct_bluestein=" << nkind[4] << " ("
+    << P.blue_groups_.size() << " groups) chirpz=" << nkind[3] << " (" << P.cz_groups_.size()
     << " groups) lengths=" << sched.size() << " tasks=" << P.level_count_[0] << "+"
     << P.level_count_[1]
     << " grid=" << P.grid_ << " lds=" << P.lds_
@@ -500,6 +604,21 @@ hipError_t MixedPlan::enqueue(const void *in, void *out, void *ws, hipStream_t s
     a.ntasks = level_count_[L];
     const int grid = (int)std::min<int64_t>(level_count_[L], level_grid_[L]);
     const hipError_t err = launch_mixed(prec_, L, a, grid, lds_, s);
+    if (err != hipSuccess) return err;
+  }
+  for (const auto &g : blue_groups_) {
+    KArgs k{};
+    k.in = g->from_ws ? ws : in;
+    k.out = out;
+    k.tw = g->eng.tw.p;
+    k.desc = g->eng.d();
+    k.aux0 = aux_.p;
+    k.aux1 = aux_.p;
+    k.aux2 = g->members.p;
+    k.count = g->count;
+    k.n = g->M;
+    const hipError_t err = g->c.e->launch(k, (k.count + g->c.e->tpb - 1) / g->c.e->tpb,
+                                          g->c.e->lds_bytes(g->M), s);
     if (err != hipSuccess) return err;
   }
   const size_t cb = 2 * (size_t)prec_compute_bytes(prec_);

@@ -1,6 +1,7 @@
 // mixed_plan.hpp -- plan of a batch of C2C transforms of different lengths,
-// run as at most two tile launches (one per dependency level) plus three
-// launches per group of chirp-z members.
+// run as at most two tile launches (one per dependency level), one launch
+// per group of one-block Bluestein transforms and three launches per group
+// of chirp-z members.
 #pragma once
 
 #include <memory>
@@ -24,8 +25,8 @@ class MixedPlan {
                                           const int64_t *out_off, std::string *why);
 
   // Launch the tile kernel on `s` once per non-empty dependency level (at
-  // most two launches), then three launches per chirp-z group. `ws` holds
-  // workspace_bytes() of device memory.
+  // most two launches), then one launch per Bluestein group and three per
+  // chirp-z group. `ws` holds workspace_bytes() of device memory.
   hipError_t enqueue(const void *in, void *out, void *ws, hipStream_t s) const;
   size_t workspace_bytes() const { return data_bytes_; }
   std::string describe() const { return summary_; }
@@ -56,6 +57,17 @@ class MixedPlan {
     DevMem members;  // CzMember records
   };
   std::vector<std::unique_ptr<CzGroup>> cz_groups_;
+  // one-block Bluestein transforms on the compiled fused kernel, one launch
+  // per (padded length M, source): whole members read from the input, or
+  // the rows of Cooley-Tukey members read from the workspace
+  struct BlueGroup {
+    Choice c;
+    EngineData eng;
+    int64_t M = 0, count = 0;
+    bool from_ws = false;
+    DevMem members;  // BlueMember records
+  };
+  std::vector<std::unique_ptr<BlueGroup>> blue_groups_;
   std::vector<std::unique_ptr<Node>> fixed_groups_;  // fixed point: one per length
   std::string summary_;
 };

@@ -16,10 +16,12 @@ from gpu_util import bf16_bits, bf16_to_f32, from_dev, relerr, to_dev
 pytestmark = pytest.mark.gpu
 
 # one of each schedule: direct (<= 4096, smooth), four-step (smooth > 4096),
-# Bluestein in one block (other n <= 2048), chirp-z (other n > 2048)
+# Bluestein in one block (other n <= 2049), four-step with Bluestein rows
+# (n = n1 * n2, n1 smooth, n2 <= 2049: 6054 = 3 * 2018, 16398 = 9 * 1822,
+# 65344 = 32 * 2042), chirp-z (the rest)
 CURATED = [1, 2, 3, 5, 7, 8, 16, 17, 23, 29, 31, 58, 64, 97, 100, 127, 243, 256, 1000,
-           1009, 2047, 2048, 2053, 4096, 4097, 4099, 5000, 8192, 15015, 16384, 16398,
-           40009, 65521, 65536, 65537, 131072]
+           1009, 2047, 2048, 2053, 4096, 4097, 4099, 5000, 6054, 8192, 15015, 16384, 16398,
+           40009, 65344, 65521, 65536, 65537, 131072]
 # the single-plan bounds: f16 1e-3 (2^-11 output rounding), bf16 8e-3 (2^-8)
 TOL_MIX = {"f32": 2e-6, "f64": 1e-12, "f16": 1e-3, "bf16": 8e-3}
 
@@ -63,7 +65,7 @@ def _run(torch, lengths, direction=P.PLFFT_FORWARD, dtype="f32", seed=1):
 def test_mixed_curated_f32(orc, torch_cuda, direction):
     plan, x, y = _run(torch_cuda, CURATED, direction, "f32", seed=11)
     d = plan.describe()
-    for tag in ("direct=", "fourstep=", "bluestein=", "chirpz="):
+    for tag in (" direct=", " fourstep=", " bluestein=", " ct_bluestein=", " chirpz="):
         assert tag in d and f"{tag}0 " not in d, d
     ref = _oracle_members(orc, x, CURATED, _offsets(CURATED), direction)
     _check(y, ref, CURATED, _offsets(CURATED), TOL_MIX["f32"])

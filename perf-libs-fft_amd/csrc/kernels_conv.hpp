@@ -19,6 +19,48 @@
 
 namespace plf {
 
+// Between the two transforms of a static-chain engine: B is loaded before
+// the first transform at the slots where the second one's stage 0 takes its
+// inputs (butterfly b = t + k TPT, element b + m N / R0), and one LDS read
+// turns the first transform's natural-order output into those inputs,
+// times B and swapped; `first` sees element 0. The caller puts a barrier
+// after it and runs the second transform from registers.
+template <class Eng>
+struct ConvMid {
+  using C = typename Eng::C;
+  using Ch = typename Eng::Ch;
+  static constexpr int TPT = Eng::TPT;
+  static constexpr int R0 = Ch::radix(0);
+  static constexpr int NB0 = Ch::N / R0;
+  static constexpr int KB0 = (NB0 + TPT - 1) / TPT;
+  static constexpr int SLOTS = KB0 * R0;
+  static __device__ __forceinline__ bool act(int t, int k) {
+    return (NB0 % TPT == 0) || (t + k * TPT < NB0);
+  }
+  static __device__ __forceinline__ void load_B(const cpx<C> *B, int t, bool valid,
+                                                cpx<C> (&bv)[SLOTS]) {
+#pragma unroll
+    for (int k = 0; k < KB0; ++k)
+#pragma unroll
+      for (int m = 0; m < R0; ++m)
+        bv[k * R0 + m] = (act(t, k) && valid) ? B[t + k * TPT + m * NB0] : cpx<C>{C(0), C(0)};
+  }
+  template <class First>
+  static __device__ __forceinline__ void gather(const cpx<C> *lds, int t,
+                                                const cpx<C> (&bv)[SLOTS], cpx<C> (&vin)[SLOTS],
+                                                First &&first) {
+#pragma unroll
+    for (int k = 0; k < KB0; ++k)
+#pragma unroll
+      for (int m = 0; m < R0; ++m) {
+        const cpx<C> A = act(t, k) ? lds[lds_pad<Eng::PAD>(t + k * TPT + m * NB0)]
+                                   : cpx<C>{C(0), C(0)};
+        if (k == 0 && m == 0 && t == 0) first(A);
+        vin[k * R0 + m] = cswap(cmul(A, bv[k * R0 + m]));
+      }
+  }
+};
+
 // aux0: B (M entries), aux1: chirp c (n entries); a.n = M, a.n1 = n.
 // Gathered launch (aux2 = BlueMember records, one per transform; a mixed
 // batch's members and rows of equal M): length, offsets, output stride and
@@ -44,28 +86,36 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_bluestein(KArgs a) {
   const cpx<C> *B = static_cast<const cpx<C> *>(a.aux0) + (mt ? rec->B_off : 0);
   const cpx<C> *chirp = static_cast<const cpx<C> *>(a.aux1) + (mt ? rec->chirp_off : 0);
   const int64_t is = mt ? 1 : a.istride, os = mt ? rec->ostride : a.ostride;
-  Eng::template run<true>(
-      a.desc, lds, tw, t, valid,
-      [&](int e) {
-        if (e >= n) return cpx<C>{C(0), C(0)};
-        cpx<C> v = ST::ld(in[(int64_t)e * is]);
-        if (BWD) v = cswap(v);
-        return cmul(v, chirp[e]);
-      },
-      [&](int, cpx<C>) {});
-  for (int e = t; e < M; e += TPT) {
-    const int p = lds_pad<Eng::PAD>(e);
-    lds[p] = cswap(cmul(lds[p], B[e]));
+  auto load = [&](int e) {
+    if (e >= n) return cpx<C>{C(0), C(0)};
+    cpx<C> v = ST::ld(in[(int64_t)e * is]);
+    if (BWD) v = cswap(v);
+    return cmul(v, chirp[e]);
+  };
+  auto store = [&](int e, cpx<C> v) {
+    if (e < n) {
+      cpx<C> r = cmul(cswap(v), chirp[e]);
+      out[(int64_t)e * os] = ST::st(BWD ? cswap(r) : r);
+    }
+  };
+  if constexpr (Eng::NFIX > 0) {
+    using Mid = ConvMid<Eng>;
+    cpx<C> bv[Mid::SLOTS], vin[Mid::SLOTS];
+    Mid::load_B(B, t, valid, bv);
+    Eng::template run<true>(a.desc, lds, tw, t, valid, load, [&](int, cpx<C>) {});
+    Mid::gather(lds, t, bv, vin, [](cpx<C>) {});
+    __syncthreads();  // the second transform's first exchange rewrites LDS
+    Eng::E::run_regs(vin, lds, tw, t, store, valid);
+  } else {
+    Eng::template run<true>(a.desc, lds, tw, t, valid, load, [&](int, cpx<C>) {});
+    for (int e = t; e < M; e += TPT) {
+      const int p = lds_pad<Eng::PAD>(e);
+      lds[p] = cswap(cmul(lds[p], B[e]));
+    }
+    __syncthreads();
+    Eng::template run<false, true>(
+        a.desc, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; }, store);
   }
-  __syncthreads();
-  Eng::template run<false, true>(
-      a.desc, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; },
-      [&](int e, cpx<C> v) {
-        if (e < n) {
-          cpx<C> r = cmul(cswap(v), chirp[e]);
-          out[(int64_t)e * os] = ST::st(BWD ? cswap(r) : r);
-        }
-      });
 }
 
 // aux0: B (m entries), aux1: perm (int32, m entries); a.n = m, a.n1 = n.
@@ -92,30 +142,33 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_rader(KArgs a) {
     x0 = ST::ld(in[0]);
     if (BWD) x0 = cswap(x0);
   }
-  Eng::template run<true>(
-      a.desc, lds, tw, t, valid,
-      [&](int e) {
-        cpx<C> v = ST::ld(in[(int64_t)perm[e] * is]);
-        return BWD ? cswap(v) : v;
-      },
-      [&](int, cpx<C>) {});
-  for (int e = t; e < m; e += TPT) {
-    const int p = lds_pad<Eng::PAD>(e);
-    const cpx<C> A = lds[p];
-    if (e == 0 && valid) {
+  auto load = [&](int e) {
+    cpx<C> v = ST::ld(in[(int64_t)perm[e] * is]);
+    return BWD ? cswap(v) : v;
+  };
+  auto store = [&](int e, cpx<C> v) {
+    const int k = perm[e == 0 ? 0 : m - e];
+    const cpx<C> r = cadd(x0, cswap(v));
+    out[(int64_t)k * os] = ST::st(BWD ? cswap(r) : r);
+  };
+  auto first = [&](cpx<C> A) {  // X_0 = x_0 + A_0
+    if (valid) {
       const cpx<C> y0 = cadd(x0, A);
       out[0] = ST::st(BWD ? cswap(y0) : y0);
     }
+  };
+  // (the register-resident middle step of k_bluestein measured slower here:
+  // the Rader chains 4.8.3 and 16.3.3.7 are already register-bound)
+  Eng::template run<true>(a.desc, lds, tw, t, valid, load, [&](int, cpx<C>) {});
+  for (int e = t; e < m; e += TPT) {
+    const int p = lds_pad<Eng::PAD>(e);
+    const cpx<C> A = lds[p];
+    if (e == 0) first(A);
     lds[p] = cswap(cmul(A, B[e]));
   }
   __syncthreads();
   Eng::template run<false, true>(
-      a.desc, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; },
-      [&](int e, cpx<C> v) {
-        const int k = perm[e == 0 ? 0 : m - e];
-        const cpx<C> r = cadd(x0, cswap(v));
-        out[(int64_t)k * os] = ST::st(BWD ? cswap(r) : r);
-      });
+      a.desc, lds, tw, t, valid, [&](int) { return cpx<C>{C(0), C(0)}; }, store);
 }
 
 }  // namespace plf

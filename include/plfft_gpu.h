@@ -82,9 +82,12 @@ const char *plfft_last_error(void);
  * Mixed-length batches: one plan transforms `count` complex-to-complex
  * members of different lengths (powers of two, other smooth lengths and
  * primes alike) in a fixed number of kernel launches: at most two for the
- * one-pass and four-step members (one per dependency level) and three per
- * group of chirp-z members of equal padded length (at most five groups for
- * lengths up to 65536). The reference has no such object: a
+ * one-pass and four-step members (one per dependency level), one per group
+ * of one-block Bluestein transforms of equal padded length (whole members,
+ * and the rows of members split as n1 x n2 with a large prime factor in n2;
+ * at most seven groups per kind), and three per group of chirp-z members of
+ * equal padded length (at most ten groups for lengths up to 65536). The
+ * reference has no such object: a
  * reference caller creates one plan per length (plfft_plan_create_from_config,
  * include/plfft.h:491-501) and executes them one after another; this plan
  * replaces that loop.

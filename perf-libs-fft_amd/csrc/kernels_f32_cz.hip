@@ -13,12 +13,16 @@ void register_cz_f32_compute(std::vector<KEntry> &r) {
   PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,    8,  16, 8, 8);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,    8,  16, 8, 8);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,    8,  16, 16, 8);
-  PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   16,  32, 16, 16);
+  // 256: 32 threads x 8 points (42 VGPRs) measured 3 % faster than 16 x 16;
+  // the chirp-out, Rader and row passes measured 2-3 % slower that way
+  PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   32,  16, 8, 8, 4);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   32,  16, 16, 16, 2);
   PLF_STATIC(k_cz_col_in, KK_CZ_COL_IN, S, P,   64,  16, 16, 16, 4);
   PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,    8,  16, 16, 8);
   PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   16,  32, 16, 16);
-  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   32,  16, 16, 16, 2);
+  // 512: 64 threads x 8 points (70 VGPRs, 7 waves per SIMD) measured 2 %
+  // faster than 32 x 16 (16, 16, 2)
+  PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   64,   8, 8, 8, 8);
   PLF_STATIC(k_cz_row_mid, KK_CZ_ROW_MID, S, P,   64,  16, 16, 16, 4);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,    8,  16, 16, 8);
   PLF_STATIC(k_cz_col_out, KK_CZ_COL_OUT, S, P,   16,  32, 16, 16);

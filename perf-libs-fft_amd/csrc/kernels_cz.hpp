@@ -31,18 +31,6 @@
 
 namespace plf {
 
-// Logical block index that keeps consecutive indices on one XCD. The
-// dispatcher hands block b to XCD b mod 8; with this order the column tiles
-// of one member, which share the cache lines at their edges (and, for
-// Rader's permuted columns, every line of the member's input), run in one L2.
-// Block b is the (b >> 3)-th block of XCD x = b & 7, and XCD x owns the
-// q + [x < r] logical indices after those of XCDs 0..x-1 (q = G >> 3,
-// r = G & 7), so the map is a bijection of [0, G).
-__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t G) {
-  const int64_t x = b & 7, q = G >> 3, r = G & 7;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + (b >> 3);
-}
-
 template <typename C>
 __device__ __forceinline__ cpx<C> cz_twiddle(const cpx<C> *__restrict__ split, int shift,
                                              int64_t e) {

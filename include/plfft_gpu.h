@@ -100,7 +100,7 @@ const char *plfft_last_error(void);
  * to 131072 are accepted for any n, and up to 262144 for n whose prime
  * factors are all at most 7. data_type: FP32, FP64, FP16 or BF16 (FP32
  * arithmetic for the 16-bit types), or Q15 / Q7 with power-of-two lengths up
- * to 32768 (one launch per distinct length; other lengths fail, as the
+ * to 32768 (one launch per kernel tier, at most five; other lengths fail, as the
  * reference's fixed-point plans do).
  * ---------------------------------------------------------------------- */
 typedef struct plfft_mixed_plan_t plfft_mixed_plan_t;

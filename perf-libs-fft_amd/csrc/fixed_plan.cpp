@@ -55,49 +55,51 @@ void push_tw(std::vector<int8_t> &tab, int bits, int dir, int64_t n, int64_t h, 
 
 bool pow2(int64_t n) { return n >= 1 && (n & (n - 1)) == 0; }
 
+// The descriptor of n's chain and its stage twiddles (the twiddles of input
+// m of column j at (m-1) L + j of each stage s >= 1, for this direction),
+// appended to `tab`; the descriptor's offsets are relative to where the
+// table starts.
+GenericDesc fixed_tables(int bits, int64_t n, bool bwd, std::vector<int8_t> *tab) {
+  std::vector<int> chain;
+  if (n > 1) chain = dynamic_chain(n);
+  GenericDesc d{};
+  d.n = (int)n;
+  d.ns = (int)chain.size();
+  const int dir = bwd ? 1 : -1;
+  int64_t L = 1, off = 0;
+  for (int s = 0; s < d.ns; ++s) {
+    const int R = chain[s];
+    d.radix[s] = R;
+    d.L[s] = (int)L;
+    d.twoff[s] = (int)off;
+    if (s > 0) {
+      for (int m = 1; m < R; ++m)
+        for (int64_t j = 0; j < L; ++j) push_tw(*tab, bits, dir, L * R, m, j);
+      off += L * (R - 1);
+    }
+    L *= R;
+  }
+  return d;
+}
+
+// The kernel tier of length n (the first whose longest length covers it).
+int fixed_tier(int64_t n) {
+  for (int t = 0; t < kNumFixedTiers; ++t)
+    if (n <= kFixedTiers[t].max_n) return t;
+  return -1;
+}
+
 class FixedC2CNode : public Node {
  public:
   FixedC2CNode(int bits, int64_t n, bool bwd, int64_t batch, int64_t is, int64_t id, int64_t os,
                int64_t od)
       : bits_(bits), n_(n), bwd_(bwd), batch_(batch), is_(is), id_(id), os_(os), od_(od) {}
 
-  // members at explicit offsets (complex elements), unit element stride
-  hipError_t set_offsets(const std::vector<int64_t> &in_off, const std::vector<int64_t> &out_off) {
-    hipError_t err = ioff_.upload(in_off.data(), in_off.size() * sizeof(int64_t));
-    if (err == hipSuccess) err = ooff_.upload(out_off.data(), out_off.size() * sizeof(int64_t));
-    return err;
-  }
-
   hipError_t init() {
-    tier_ = -1;
-    for (int t = 0; t < kNumFixedTiers; ++t)
-      if (n_ <= kFixedTiers[t].max_n) {
-        tier_ = t;
-        break;
-      }
+    tier_ = fixed_tier(n_);
     if (tier_ < 0) return hipErrorInvalidValue;
-    std::vector<int> chain;
-    if (n_ > 1) chain = dynamic_chain(n_);
-    GenericDesc d{};
-    d.n = (int)n_;
-    d.ns = (int)chain.size();
-    // table: the stage twiddles of input m of column j at (m-1) L + j of
-    // each stage s >= 1, for this direction (dir = +1 backward)
     std::vector<int8_t> tab;
-    const int dir = bwd_ ? 1 : -1;
-    int64_t L = 1, off = 0;
-    for (int s = 0; s < d.ns; ++s) {
-      const int R = chain[s];
-      d.radix[s] = R;
-      d.L[s] = (int)L;
-      d.twoff[s] = (int)off;
-      if (s > 0) {
-        for (int m = 1; m < R; ++m)
-          for (int64_t j = 0; j < L; ++j) push_tw(tab, bits_, dir, L * R, m, j);
-        off += L * (R - 1);
-      }
-      L *= R;
-    }
+    const GenericDesc d = fixed_tables(bits_, n_, bwd_, &tab);
     hipError_t err = desc_.upload(&d, sizeof(d));
     if (err == hipSuccess && !tab.empty()) err = tab_.upload(tab.data(), tab.size());
     return err;
@@ -121,9 +123,7 @@ class FixedC2CNode : public Node {
     a.ostride = os_;
     a.odist = od_;
     a.n = n_;
-    a.flags = (bwd_ ? 1 : 0) | (ioff_.p ? 2 : 0);
-    a.aux0 = ioff_.p;
-    a.aux1 = ooff_.p;
+    a.flags = bwd_ ? 1 : 0;
     const int64_t grid = (count + t.tpb - 1) / t.tpb;
     const size_t lds =
         (size_t)t.tpb * (size_t)(n_ + (n_ >> 5) + 1) * (bits_ == 15 ? 4 : 2);
@@ -144,7 +144,7 @@ class FixedC2CNode : public Node {
   bool bwd_;
   int64_t batch_, is_, id_, os_, od_;
   int tier_ = -1;
-  DevMem desc_, tab_, ioff_, ooff_;
+  DevMem desc_, tab_;
 };
 
 // R2C (forward) or C2R (backward) through a complex staging buffer.
@@ -249,21 +249,122 @@ std::unique_ptr<Node> build_fixed_node(const Problem &p, std::string *why) {
 
 // Fixed-point members of one power-of-two length at explicit offsets (the
 // groups of a fixed-point mixed-length batch).
-std::unique_ptr<Node> build_fixed_gather_node(int bits, int64_t n, bool bwd,
-                                              const std::vector<int64_t> &in_off,
-                                              const std::vector<int64_t> &out_off,
-                                              std::string *why) {
-  if (!pow2(n) || n > kFixedTiers[kNumFixedTiers - 1].max_n) {
-    *why = "fixed-point mixed batches need power-of-two lengths <= 32768";
-    return nullptr;
+// A fixed-point mixed-length batch: one gathered launch per kernel tier.
+// Members are sorted by (tier, length); each block takes up to the tier's
+// members-per-block of one length (FxBlock records), so every block runs a
+// single stage sequence.
+class FixedMixedNode : public Node {
+ public:
+  FixedMixedNode(int bits, bool bwd) : bits_(bits), bwd_(bwd) {}
+
+  hipError_t init(const std::vector<int64_t> &n, const std::vector<int64_t> &in_off,
+                  const std::vector<int64_t> &out_off) {
+    const size_t count = n.size();
+    std::vector<size_t> order(count);
+    for (size_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return n[x] < n[y]; });
+    std::vector<int64_t> ioff(count), ooff(count);
+    std::vector<GenericDesc> descs;
+    std::vector<int8_t> tab;
+    const size_t tw_bytes = bits_ == 15 ? 8 : 4;  // one QTw entry
+    std::vector<FxBlock> blocks[kNumFixedTiers];
+    size_t i = 0;
+    while (i < count) {
+      const int64_t len = n[order[i]];
+      size_t j = i;
+      while (j < count && n[order[j]] == len) ++j;
+      const int tier = fixed_tier(len);
+      if (tier < 0) return hipErrorInvalidValue;
+      const int64_t tw_off = (int64_t)(tab.size() / tw_bytes);
+      descs.push_back(fixed_tables(bits_, len, bwd_, &tab));
+      ++lengths_;
+      const int tpb = kFixedTiers[tier].tpb;
+      for (size_t b = i; b < j; b += tpb) {
+        FxBlock r{};
+        r.desc = (int32_t)descs.size() - 1;
+        r.count = (int32_t)std::min<size_t>(tpb, j - b);
+        r.tw_off = tw_off;
+        r.member = (int64_t)b;
+        blocks[tier].push_back(r);
+      }
+      max_n_[tier] = std::max(max_n_[tier], len);
+      for (size_t k = i; k < j; ++k) {
+        ioff[k] = in_off[order[k]];
+        ooff[k] = out_off[order[k]];
+      }
+      i = j;
+    }
+    hipError_t err = descs_.upload(descs.data(), descs.size() * sizeof(GenericDesc));
+    if (err == hipSuccess && !tab.empty()) err = tab_.upload(tab.data(), tab.size());
+    if (err == hipSuccess) err = ioff_.upload(ioff.data(), count * sizeof(int64_t));
+    if (err == hipSuccess) err = ooff_.upload(ooff.data(), count * sizeof(int64_t));
+    for (int t = 0; t < kNumFixedTiers && err == hipSuccess; ++t) {
+      nblocks_[t] = (int64_t)blocks[t].size();
+      if (nblocks_[t]) err = blocks_[t].upload(blocks[t].data(), blocks[t].size() * sizeof(FxBlock));
+    }
+    return err;
   }
-  auto node = std::make_unique<FixedC2CNode>(bits, n, bwd, (int64_t)in_off.size(), 1, 0, 1, 0);
-  hipError_t err = node->init();
-  if (err == hipSuccess) err = node->set_offsets(in_off, out_off);
+
+  hipError_t enqueue(const void *in, void *out, void *, hipStream_t s) const override {
+    for (int t = 0; t < kNumFixedTiers; ++t) {
+      if (!nblocks_[t]) continue;
+      KArgs a{};
+      a.in = in;
+      a.out = out;
+      a.tw = tab_.p;
+      a.desc = static_cast<const GenericDesc *>(descs_.p);
+      a.aux0 = ioff_.p;
+      a.aux1 = ooff_.p;
+      a.aux2 = blocks_[t].p;
+      a.istride = a.ostride = 1;
+      a.n = max_n_[t];
+      a.flags = (bwd_ ? 1 : 0) | 2 | 4;
+      const size_t lds = (size_t)kFixedTiers[t].tpb * (size_t)(max_n_[t] + (max_n_[t] >> 5) + 1) *
+                         (bits_ == 15 ? 4 : 2);
+      const hipError_t err = launch_fixed(bits_, t, a, nblocks_[t], lds, s);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+
+  int launches() const {
+    int k = 0;
+    for (int t = 0; t < kNumFixedTiers; ++t) k += nblocks_[t] ? 1 : 0;
+    return k;
+  }
+  int lengths() const { return lengths_; }
+  std::string describe() const override {
+    std::ostringstream o;
+    o << "(fixed_q" << bits_ << "_gather lengths=" << lengths_ << " launches=" << launches()
+      << ")";
+    return o.str();
+  }
+
+ private:
+  int bits_;
+  bool bwd_;
+  int lengths_ = 0;
+  int64_t max_n_[kNumFixedTiers] = {}, nblocks_[kNumFixedTiers] = {};
+  DevMem descs_, tab_, ioff_, ooff_, blocks_[kNumFixedTiers];
+};
+
+std::unique_ptr<Node> build_fixed_mixed_node(int bits, bool bwd, const std::vector<int64_t> &n,
+                                             const std::vector<int64_t> &in_off,
+                                             const std::vector<int64_t> &out_off,
+                                             int *launches, std::string *why) {
+  for (int64_t len : n)
+    if (!pow2(len) || len > kFixedTiers[kNumFixedTiers - 1].max_n) {
+      *why = "fixed-point mixed batches need power-of-two lengths <= 32768";
+      return nullptr;
+    }
+  auto node = std::make_unique<FixedMixedNode>(bits, bwd);
+  const hipError_t err = node->init(n, in_off, out_off);
   if (err != hipSuccess) {
     *why = std::string("fixed-point table setup failed: ") + hipGetErrorString(err);
     return nullptr;
   }
+  *launches = node->launches();
   return node;
 }
 

@@ -57,4 +57,14 @@ struct BlueMember {
   int64_t chirp_off, B_off;
 };
 
+// One block of a gathered fixed-point launch (kernels_fixed.hpp): `count`
+// members from sorted member index `member` on, all of one length, whose
+// descriptor is desc-th in the launch's array and whose stage twiddles start
+// tw_off table entries into the launch's table. One length per block keeps
+// every thread of a block on the same stage sequence (the same barriers).
+struct FxBlock {
+  int32_t desc, count;
+  int64_t tw_off, member;
+};
+
 }  // namespace plf

@@ -291,16 +291,21 @@ struct FixedEngine {
 // (storage pairs), tw = the stage twiddles (QTw, for the plan's direction),
 // desc, count, strides and distances in complex elements, n; flags bit1:
 // member offsets (complex elements) from the tables aux0 (input) and aux1
-// (output) instead of member * distance.
+// (output) instead of member * distance; flags bit2 (with bit1, unit
+// strides): a gathered launch of several lengths, aux2 = one FxBlock per
+// block (desc and tw are then the arrays the records index; n = the longest
+// length, which sizes each member's LDS).
 template <int BITS, int TPT, int TPB, int PTS, bool BWD>
 __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   using P = QPair<BITS>;
   extern __shared__ unsigned char plf_dyn_lds[];
   const int tb = threadIdx.x / TPT, t = threadIdx.x % TPT;
-  const int64_t member = (int64_t)blockIdx.x * TPB + tb;
-  const bool valid = member < a.count;
+  const FxBlock *rec = (a.flags & 4) ? static_cast<const FxBlock *>(a.aux2) + blockIdx.x : nullptr;
+  const int64_t member = rec ? rec->member + tb : (int64_t)blockIdx.x * TPB + tb;
+  const bool valid = rec ? tb < rec->count : member < a.count;
+  const GenericDesc *desc = rec ? a.desc + rec->desc : a.desc;
   P *lds = reinterpret_cast<P *>(plf_dyn_lds) + tb * lds_padded<5>((int)a.n);
-  const QTw<BITS> *tab = static_cast<const QTw<BITS> *>(a.tw);
+  const QTw<BITS> *tab = static_cast<const QTw<BITS> *>(a.tw) + (rec ? rec->tw_off : 0);
   const bool table = a.flags & 2;
   const int64_t ioff = !valid ? 0 : table ? static_cast<const int64_t *>(a.aux0)[member]
                                           : member * a.idist;
@@ -316,7 +321,7 @@ __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   auto go = [&](auto unit) {
     constexpr bool U = decltype(unit)::value;
     E::run(
-        a.desc, lds, tab, t, valid,
+        desc, lds, tab, t, valid,
         [&](int e) { return O::in(in[U ? (int64_t)e : e * is]); },
         [&](int e, V v) { out[U ? (int64_t)e : e * os] = O::out(v); });
   };

@@ -229,27 +229,28 @@ std::unique_ptr<MixedPlan> MixedPlan::build(int prec, int fixed_bits, int dir, i
   P.count_ = count;
   if (fixed_bits) {
     // The reference's fixed-point plans are power-of-two only
-    // (src/planner/planner.cpp:38-47): one gather launch per length.
-    std::map<int64_t, std::pair<std::vector<int64_t>, std::vector<int64_t>>> groups;
+    // (src/planner/planner.cpp:38-47): one gathered launch per kernel tier.
+    std::vector<int64_t> ns((size_t)count), io((size_t)count), oo((size_t)count);
+    std::map<int64_t, int> distinct;
     int64_t ipos = 0, opos = 0;
     for (int64_t i = 0; i < count; ++i) {
-      const int64_t io = in_off ? in_off[i] : ipos, oo = out_off ? out_off[i] : opos;
+      ns[(size_t)i] = n[i];
+      io[(size_t)i] = in_off ? in_off[i] : ipos;
+      oo[(size_t)i] = out_off ? out_off[i] : opos;
       ipos += n[i];
       opos += n[i];
-      P.in_span_ = std::max(P.in_span_, io + n[i]);
-      P.out_span_ = std::max(P.out_span_, oo + n[i]);
-      groups[n[i]].first.push_back(io);
-      groups[n[i]].second.push_back(oo);
+      P.in_span_ = std::max(P.in_span_, io[(size_t)i] + n[i]);
+      P.out_span_ = std::max(P.out_span_, oo[(size_t)i] + n[i]);
+      distinct[n[i]] = 1;
     }
-    for (const auto &g : groups) {
-      std::unique_ptr<Node> node = build_fixed_gather_node(fixed_bits, g.first, dir > 0,
-                                                           g.second.first, g.second.second, why);
-      if (!node) return nullptr;
-      P.fixed_groups_.push_back(std::move(node));
-    }
+    int launches = 0;
+    std::unique_ptr<Node> node =
+        build_fixed_mixed_node(fixed_bits, dir > 0, ns, io, oo, &launches, why);
+    if (!node) return nullptr;
+    P.fixed_groups_.push_back(std::move(node));
     std::ostringstream o;
-    o << "(mixed_fixed_q" << fixed_bits << " count=" << count << " lengths=" << groups.size()
-      << " launches=" << groups.size() << ")";
+    o << "(mixed_fixed_q" << fixed_bits << " count=" << count << " lengths=" << distinct.size()
+      << " launches=" << launches << ")";
     P.summary_ = o.str();
     return plan;
   }

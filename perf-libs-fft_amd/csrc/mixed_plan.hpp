@@ -19,7 +19,7 @@ class MixedPlan {
   // at out_off[i] (complex elements of the storage type). A null offset
   // array packs the members back to back. dir: -1 forward, +1 backward.
   // fixed_bits 15 / 7: Q0.15 / Q0.7 storage (power-of-two n <= 32768, one
-  // launch per distinct length); 0: floating point (prec).
+  // launch per kernel tier); 0: floating point (prec).
   static std::unique_ptr<MixedPlan> build(int prec, int fixed_bits, int dir, int64_t count,
                                           const int64_t *n, const int64_t *in_off,
                                           const int64_t *out_off, std::string *why);
@@ -68,7 +68,7 @@ class MixedPlan {
     DevMem members;  // BlueMember records
   };
   std::vector<std::unique_ptr<BlueGroup>> blue_groups_;
-  std::vector<std::unique_ptr<Node>> fixed_groups_;  // fixed point: one per length
+  std::vector<std::unique_ptr<Node>> fixed_groups_;  // fixed point: one gathered node
   std::string summary_;
 };
 

@@ -85,12 +85,13 @@ std::unique_ptr<Node> build_real_node(const Problem &p, std::string *why);
 std::unique_ptr<Node> build_r2r_node(const Problem &p, std::string *why);
 // Q0.15 / Q0.7 problems (p.fixed_bits != 0): power-of-two n only.
 std::unique_ptr<Node> build_fixed_node(const Problem &p, std::string *why);
-// Fixed-point C2C members of one power-of-two length at explicit offsets
-// (complex elements, unit element stride).
-std::unique_ptr<Node> build_fixed_gather_node(int bits, int64_t n, bool bwd,
-                                              const std::vector<int64_t> &in_off,
-                                              const std::vector<int64_t> &out_off,
-                                              std::string *why);
+// Fixed-point C2C members of power-of-two lengths n[i] at explicit offsets
+// (complex elements, unit element stride): one gathered launch per kernel
+// tier; *launches receives their number.
+std::unique_ptr<Node> build_fixed_mixed_node(int bits, bool bwd, const std::vector<int64_t> &n,
+                                             const std::vector<int64_t> &in_off,
+                                             const std::vector<int64_t> &out_off,
+                                             int *launches, std::string *why);
 
 // ---- planner internals shared by the node builders ----------------------
 // A kernel choice for one engine length: the registry entry plus the chain.

@@ -168,20 +168,23 @@ def test_fixed_unsupported_lengths(torch_cuda, n, msg):
 def test_fixed_mixed_batch(orc, torch_cuda, bits):
     """A fixed-point mixed-length batch (power-of-two lengths, the
     reference's fixed-point domain) equals one fixed-point plan per member,
-    bit for bit."""
+    bit for bit, in both directions: one gathered launch per kernel tier,
+    blocks of one length each, lengths interleaved in the batch."""
     name, npdt = DT[bits]
     rngl = np.random.default_rng(3)
-    lengths = [int(v) for v in 2 ** rngl.integers(0, 16, size=40)] + [1, 2, 32768]
+    lengths = [int(v) for v in 2 ** rngl.integers(0, 16, size=40)] + [1, 2, 32768, 1, 16, 16]
     offs = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(int)
     x = _rand(200 + bits, (sum(lengths), 2), bits)
-    plan = P.MixedPlan(lengths, P.PLFFT_FORWARD, name)
-    assert "mixed_fixed" in plan.describe()
-    dx = to_dev(torch_cuda, x.reshape(-1))
-    dy = torch_cuda.zeros_like(dx)
-    plan.execute(dx, dy)
-    y = from_dev(torch_cuda, dy).reshape(-1, 2).astype(np.int32)
-    for n, o in zip(lengths, offs):
-        ref = orc.fixed_c2c(x[o:o + n].astype(np.int32), -1, bits)
-        assert np.array_equal(y[o:o + n], ref), n
+    for direction in (P.PLFFT_FORWARD, P.PLFFT_BACKWARD):
+        plan = P.MixedPlan(lengths, direction, name)
+        d = plan.describe()
+        assert "mixed_fixed" in d and "launches=5" in d, d
+        dx = to_dev(torch_cuda, x.reshape(-1))
+        dy = torch_cuda.zeros_like(dx)
+        plan.execute(dx, dy)
+        y = from_dev(torch_cuda, dy).reshape(-1, 2).astype(np.int32)
+        for n, o in zip(lengths, offs):
+            ref = orc.fixed_c2c(x[o:o + n].astype(np.int32), direction, bits)
+            assert np.array_equal(y[o:o + n], ref), (n, direction)
     with pytest.raises(P.PlfftError):
         P.MixedPlan([16, 12], P.PLFFT_FORWARD, name)

@@ -100,6 +100,11 @@ class FixedC2CNode : public Node {
     if (tier_ < 0) return hipErrorInvalidValue;
     std::vector<int8_t> tab;
     const GenericDesc d = fixed_tables(bits_, n_, bwd_, &tab);
+    // n = 4096 over 16 x 16 x 16 with unit strides: the compile-time kernel
+    launch_tier_ = tier_;
+    if (n_ == 4096 && is_ == 1 && os_ == 1 && d.ns == 3 && d.radix[0] == 16 &&
+        d.radix[1] == 16 && d.radix[2] == 16)
+      launch_tier_ = kFixedTier4096;
     hipError_t err = desc_.upload(&d, sizeof(d));
     if (err == hipSuccess && !tab.empty()) err = tab_.upload(tab.data(), tab.size());
     return err;
@@ -127,14 +132,15 @@ class FixedC2CNode : public Node {
     const int64_t grid = (count + t.tpb - 1) / t.tpb;
     const size_t lds =
         (size_t)t.tpb * (size_t)(n_ + (n_ >> 5) + 1) * (bits_ == 15 ? 4 : 2);
-    return launch_fixed(bits_, tier_, a, grid, lds, s);
+    return launch_fixed(bits_, launch_tier_, a, grid, lds, s);
   }
 
   std::string describe() const override {
     const FixedTier &t = kFixedTiers[tier_];
     std::ostringstream o;
     o << "(fixed_q" << bits_ << "_" << (bwd_ ? "bwd" : "fwd") << " n=" << n_ << " tpt=" << t.tpt
-      << " tpb=" << t.tpb << " batch=" << batch_ << ")";
+      << " tpb=" << t.tpb << (launch_tier_ == kFixedTier4096 ? " chain=16x16x16" : "")
+      << " batch=" << batch_ << ")";
     return o.str();
   }
 
@@ -143,7 +149,7 @@ class FixedC2CNode : public Node {
   int64_t n_;
   bool bwd_;
   int64_t batch_, is_, id_, os_, od_;
-  int tier_ = -1;
+  int tier_ = -1, launch_tier_ = -1;
   DevMem desc_, tab_;
 };
 

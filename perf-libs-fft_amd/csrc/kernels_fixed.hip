@@ -26,6 +26,8 @@ hipError_t launch_dir(int tier, const KArgs &a, int64_t grid, size_t lds, hipStr
     case 2: return launch_q<k_fixed<BITS, 256, 1, 16, BWD>, 256>(a, grid, lds, s);
     case 3: return launch_q<k_fixed<BITS, 1024, 1, 16, BWD>, 1024>(a, grid, lds, s);
     case 4: return launch_q<k_fixed<BITS, 1024, 1, 32, BWD>, 1024>(a, grid, lds, s);
+    // n = 4096 on the compile-time chain 16 x 16 x 16 (unit strides)
+    case kFixedTier4096: return launch_q<k_fixed_static<BITS, 256, 1, 16, BWD, 16, 16, 16>, 256>(a, grid, lds, s);
     default: return hipErrorInvalidValue;
   }
 }

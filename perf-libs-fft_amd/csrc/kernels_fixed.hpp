@@ -200,20 +200,40 @@ struct FixedEngine {
   using TW = QTw<BITS>;
   static constexpr int PAD = 5;
 
-  template <int R, class Load, class Store>
-  static __device__ __forceinline__ void stage(const GenericDesc *__restrict__ d, int s, P *lds,
+  // A stage's shape from the runtime descriptor ...
+  struct RtStage {
+    const GenericDesc *d;
+    int s;
+    __device__ __forceinline__ int n() const { return d->n; }
+    __device__ __forceinline__ int L() const { return d->L[s]; }
+    __device__ __forceinline__ int twoff() const { return d->twoff[s]; }
+    __device__ __forceinline__ bool first() const { return s == 0; }
+    __device__ __forceinline__ bool last() const { return s == d->ns - 1; }
+  };
+  // ... or from a compile-time chain (every index below folds to a constant)
+  template <int N_, int L_, int TWOFF_, bool FIRST_, bool LAST_>
+  struct CtStage {
+    __device__ __forceinline__ constexpr int n() const { return N_; }
+    __device__ __forceinline__ constexpr int L() const { return L_; }
+    __device__ __forceinline__ constexpr int twoff() const { return TWOFF_; }
+    __device__ __forceinline__ constexpr bool first() const { return FIRST_; }
+    __device__ __forceinline__ constexpr bool last() const { return LAST_; }
+  };
+
+  template <int R, class SP, class Load, class Store>
+  static __device__ __forceinline__ void stage(const SP &sp, P *lds,
                                                const TW *__restrict__ tw, int t, bool valid,
                                                Load &load, Store &store) {
     constexpr int KMAX = (PTS + R - 1) / R;
     constexpr int SH = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
     asm volatile("" : "+v"(t));  // keep per-stage addressing out of the stage loop
-    const int n = d->n;
+    const int n = sp.n();
     const int NB = n / R;
-    const int L = d->L[s];         // a power of two: radices 2, 4, 8, 16 only
+    const int L = sp.L();          // a power of two: radices 2, 4, 8, 16 only
     const int lgL = __builtin_ctz(L);
-    const bool first = s == 0;
-    const bool last = s == d->ns - 1;
-    const TW *tws = tw + d->twoff[s];
+    const bool first = sp.first();
+    const bool last = sp.last();
+    const TW *tws = tw + sp.twoff();
     V v[KMAX][R];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) {
@@ -276,14 +296,36 @@ struct FixedEngine {
       return;
     }
     for (int s = 0; s < ns; ++s) {
+      const RtStage sp{d, s};
       switch (d->radix[s]) {
-        case 2: stage<2>(d, s, lds, tw, t, valid, load, store); break;
-        case 4: stage<4>(d, s, lds, tw, t, valid, load, store); break;
-        case 8: stage<8>(d, s, lds, tw, t, valid, load, store); break;
-        case 16: stage<16>(d, s, lds, tw, t, valid, load, store); break;
+        case 2: stage<2>(sp, lds, tw, t, valid, load, store); break;
+        case 4: stage<4>(sp, lds, tw, t, valid, load, store); break;
+        case 8: stage<8>(sp, lds, tw, t, valid, load, store); break;
+        case 16: stage<16>(sp, lds, tw, t, valid, load, store); break;
         default: break;
       }
     }
+  }
+
+  // The same stages over a compile-time chain Rs (the product N, at least
+  // two stages); the twiddle table layout is the runtime one's.
+  template <int SI, int L, int TWOFF, int... Rs, class Load, class Store>
+  static __device__ __forceinline__ void run_static_from(P *lds, const TW *tw, int t, bool valid,
+                                                         Load &load, Store &store) {
+    constexpr int NS = sizeof...(Rs);
+    constexpr int RS[NS] = {Rs...};
+    constexpr int N = (Rs * ...);
+    constexpr int R = RS[SI];
+    const CtStage<N, L, TWOFF, SI == 0, SI == NS - 1> sp{};
+    stage<R>(sp, lds, tw, t, valid, load, store);
+    if constexpr (SI + 1 < NS)
+      run_static_from<SI + 1, L * R, (SI == 0 ? 0 : TWOFF + L * (R - 1)), Rs...>(lds, tw, t, valid,
+                                                                                   load, store);
+  }
+  template <int... Rs, class Load, class Store>
+  static __device__ __forceinline__ void run_static(P *lds, const TW *tw, int t, bool valid,
+                                                    Load &&load, Store &&store) {
+    run_static_from<0, 1, 0, Rs...>(lds, tw, t, valid, load, store);
   }
 };
 
@@ -327,6 +369,28 @@ __global__ void __launch_bounds__(TPT *TPB) k_fixed(KArgs a) {
   };
   if (is == 1 && os == 1) go(std::true_type{});
   else go(std::false_type{});
+}
+
+// k_fixed for one length with a compile-time chain Rs (unit strides,
+// members at distance idist / odist, no offset tables). KArgs as k_fixed.
+template <int BITS, int TPT, int TPB, int PTS, bool BWD, int... Rs>
+__global__ void __launch_bounds__(TPT *TPB) k_fixed_static(KArgs a) {
+  using P = QPair<BITS>;
+  extern __shared__ unsigned char plf_dyn_lds[];
+  constexpr int N = (Rs * ...);
+  const int tb = threadIdx.x / TPT, t = threadIdx.x % TPT;
+  const int64_t member = (int64_t)blockIdx.x * TPB + tb;
+  const bool valid = member < a.count;
+  P *lds = reinterpret_cast<P *>(plf_dyn_lds) + tb * lds_padded<5>(N);
+  const QTw<BITS> *tab = static_cast<const QTw<BITS> *>(a.tw);
+  const P *in = static_cast<const P *>(a.in) + (valid ? member * a.idist : 0);
+  P *out = static_cast<P *>(a.out) + (valid ? member * a.odist : 0);
+  using E = FixedEngine<BITS, TPT, PTS, BWD>;
+  using O = typename E::O;
+  using V = typename O::V;
+  E::template run_static<Rs...>(
+      lds, tab, t, valid, [&](int e) { return O::in(in[e]); },
+      [&](int e, V v) { out[e] = O::out(v); });
 }
 
 // Real <-> complex staging for fixed-point R2C / C2R through a complex

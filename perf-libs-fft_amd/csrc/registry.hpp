@@ -136,6 +136,9 @@ struct FixedTier {
   int64_t max_n;
 };
 constexpr int kNumFixedTiers = 5;
+// launch_fixed tier of the n = 4096 kernel on a compile-time chain (unit
+// strides, members at distance idist / odist)
+constexpr int kFixedTier4096 = 5;
 extern const FixedTier kFixedTiers[kNumFixedTiers];
 hipError_t launch_fixed(int bits, int tier, const KArgs &a, int64_t grid, size_t lds,
                         hipStream_t s);

@@ -126,6 +126,49 @@ __global__ void __launch_bounds__(Eng::TPT *TPB) k_c2c(KArgs a) {
       [&](int e, cpx<C> v) { out[(int64_t)e * os] = ST::st(BWD ? cswap(v) : v); });
 }
 
+// Batched C2C of a small odd prime N (3..23) with packed members (unit
+// stride, distance N): one member per thread, 256 members per block. The
+// block's 256 N consecutive elements pass through LDS on the way in and
+// out, so every global load and store instruction of a wave covers 512
+// contiguous bytes instead of 64 lanes N elements apart; the member's
+// DFT_N runs in registers (lds[t N + j] with N odd meets each bank about
+// twice per wave). KArgs: in, out, count. In place is safe: a block reads
+// all of its members before it writes any.
+template <typename S, bool BWD, int N>
+__global__ void __launch_bounds__(256) k_c2c_staged(KArgs a) {
+  using ST = StorageT<S>;
+  using E = typename ST::E;
+  using C = typename ST::C;
+  cpx<C> *lds = dyn_lds<C>();
+  const int t = threadIdx.x;
+  const int64_t m0 = (int64_t)blockIdx.x * 256;
+  const int members = (int)(a.count - m0 < 256 ? a.count - m0 : 256);
+  const int total = members * N;
+  const E *in = static_cast<const E *>(a.in) + m0 * N;
+  E *out = static_cast<E *>(a.out) + m0 * N;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int e = t + 256 * k;
+    if (e < total) lds[e] = ST::ldnt(in + e);
+  }
+  __syncthreads();
+  cpx<C> v[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const cpx<C> x = lds[t * N + j];
+    v[j] = BWD ? cswap(x) : x;
+  }
+  Dft<N, C>::run(v);
+#pragma unroll
+  for (int j = 0; j < N; ++j) lds[t * N + j] = BWD ? cswap(v[j]) : v[j];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int e = t + 256 * k;
+    if (e < total) ST::stnt(out + e, lds[e]);
+  }
+}
+
 // batched C2C, unit element stride: streaming (non-temporal) global access;
 // with a PAIR engine neighbouring elements of one thread merge into 16-byte
 // accesses.

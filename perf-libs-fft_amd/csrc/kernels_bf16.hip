@@ -45,6 +45,15 @@ void register_kernels_bf16(std::vector<KEntry> &r) {
   //              TPT  TPB PAD PAIR   radix chain
   PLF_XLANE(S, P, 16, 2, 2, 2, 2);  // cross-lane, packed members
   PLF_XLANE(S, P, 32, 2, 2, 2, 2, 2);
+  // small odd primes with packed members: LDS-staged global access
+  PLF_STAGED(S, P, 3);
+  PLF_STAGED(S, P, 5);
+  PLF_STAGED(S, P, 7);
+  PLF_STAGED(S, P, 11);
+  PLF_STAGED(S, P, 13);
+  PLF_STAGED(S, P, 17);
+  PLF_STAGED(S, P, 19);
+  PLF_STAGED(S, P, 23);
   PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
   PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
   // 4096 and 16384: the split (real, then imaginary) LDS exchange first

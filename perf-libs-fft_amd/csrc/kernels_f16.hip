@@ -58,6 +58,15 @@ void register_kernels_f16(std::vector<KEntry> &r) {
     r[r.size() - i].persist = 6;
   }
   PLF_XLANE(S, P, 32, 2, 2, 2, 2, 2);
+  // small odd primes with packed members: LDS-staged global access
+  PLF_STAGED(S, P, 3);
+  PLF_STAGED(S, P, 5);
+  PLF_STAGED(S, P, 7);
+  PLF_STAGED(S, P, 11);
+  PLF_STAGED(S, P, 13);
+  PLF_STAGED(S, P, 17);
+  PLF_STAGED(S, P, 19);
+  PLF_STAGED(S, P, 23);
   PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
   PLF_UNIT(S, P,  64,   4, 4, false, 16, 16, 4);
   // 4096 and 16384: the split (real, then imaginary) LDS exchange first

@@ -14,6 +14,15 @@ void register_kernels_f32_unit(std::vector<KEntry> &r) {
   // 0.39 for the LDS radix-8x8 kernel, so it is not registered.
   PLF_XLANE(S, P, 16, 2, 2, 2, 2);
   PLF_XLANE(S, P, 32, 2, 2, 2, 2, 2);
+  // small odd primes with packed members: LDS-staged global access
+  PLF_STAGED(S, P, 3);
+  PLF_STAGED(S, P, 5);
+  PLF_STAGED(S, P, 7);
+  PLF_STAGED(S, P, 11);
+  PLF_STAGED(S, P, 13);
+  PLF_STAGED(S, P, 17);
+  PLF_STAGED(S, P, 19);
+  PLF_STAGED(S, P, 23);
   //              TPT  TPB PAD PAIR   radix chain
   PLF_UNIT(S, P,  16,  16, 4, false, 16, 16);
   PLF_UNIT(S, P,   8,  32, 4, true,  16, 16);

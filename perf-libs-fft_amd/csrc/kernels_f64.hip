@@ -7,6 +7,15 @@ void register_kernels_f64(std::vector<KEntry> &r) {
   using S = double;
   constexpr int P = P_F64;
   //                          TPT  TPB  radix chain
+  // small odd primes with packed members: LDS-staged global access
+  PLF_STAGED(S, P, 3);
+  PLF_STAGED(S, P, 5);
+  PLF_STAGED(S, P, 7);
+  PLF_STAGED(S, P, 11);
+  PLF_STAGED(S, P, 13);
+  PLF_STAGED(S, P, 17);
+  PLF_STAGED(S, P, 19);
+  PLF_STAGED(S, P, 23);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 2);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 4);
   PLF_STATIC(k_c2c, KK_C2C, S, P,    1, 256, 8);

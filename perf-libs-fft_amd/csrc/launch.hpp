@@ -166,6 +166,19 @@ inline void add_entry(std::vector<KEntry> &r, int kind, int prec, int n,
     r[r.size() - 2].xlane = true;                                                          \
   } while (0)
 
+// Packed C2C of a small odd prime N staged through LDS (k_c2c_staged):
+// 256 members of one thread each per block.
+#define PLF_STAGED(S, PREC, N)                                                             \
+  do {                                                                                     \
+    ::plf::add_entry(r, ::plf::KK_C2C_UNIT, PREC, N, {N}, 1, 256, 0, true,                 \
+                     &::plf::launch_k<&::plf::k_c2c_staged<S, false, N>, 256>,            \
+                     &::plf::launch_k<&::plf::k_c2c_staged<S, true, N>, 256>);            \
+    for (int i_ = 1; i_ <= 2; ++i_) {                                                      \
+      r[r.size() - i_].packed = true;                                                      \
+      r[r.size() - i_].staged = true;                                                      \
+    }                                                                                      \
+  } while (0)
+
 // Unit-stride software-pipelined C2C (k_c2c_pipe), PER_CU blocks per CU.
 #define PLF_PIPE(S, PREC, TPT, PAD, PER_CU, ...)                                           \
   do {                                                                                    \

@@ -58,6 +58,7 @@ std::string KEntry::name() const {
   if (waves) o << "_w" << waves;
   if (xlane) o << "_xlane";
   if (mfma) o << "_mfma";
+  if (staged) o << "_staged";
   return o.str();
 }
 

@@ -73,6 +73,7 @@ struct KEntry {
   bool xlane = false;   // cross-lane kernel: packed members only, lane twiddle table
   bool mfma = false;    // first stage on the matrix cores
   bool packed = false;  // members contiguous at distance n only
+  bool staged = false;  // one member per thread, global access staged through LDS
   KLaunch launch;
   std::string name() const;
   // LDS bytes of one block for engine length len

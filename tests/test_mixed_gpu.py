@@ -140,7 +140,9 @@ def test_mixed_half_storage(orc, torch_cuda, dtype):
 
 
 def test_mixed_offsets_gaps_and_in_place(orc, torch_cuda):
-    lengths = [300, 17, 4096, 8192, 2053, 64]
+    # every schedule: direct, one-block Bluestein, four-step, chirp-z, and
+    # four-step with Bluestein rows (6054 = 3 * 2018, 16398 = 9 * 1822)
+    lengths = [300, 17, 4096, 8192, 2053, 64, 6054, 16398]
     # input members reversed with gaps; output members packed differently
     in_offs, pos = [], 0
     for n in lengths:

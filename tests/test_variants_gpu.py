@@ -22,6 +22,9 @@ VARIANTS = [
     (16, "n16_r2x2x2x2_t1_b256_p5_xlane"),
     (32, "n32_r2x2x2x2x2_t2_b128_p5_xlane"),
     (16, "n16_r16_t1_b256"),
+    (5, "n5_r5_t1_b256_p5_staged"),
+    (17, "n17_r17_t1_b256_p5_staged"),
+    (23, "n23_r23_t1_b256_p5_staged"),
     (32, "n32_r8x4_t4_b64"),
     (64, "n64_r16x4_t4_b64"),
     (256, "n256_r16x16_t16_b16_p4"),

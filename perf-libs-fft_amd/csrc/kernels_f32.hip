@@ -44,6 +44,8 @@ void register_kernels_f32(std::vector<KEntry> &r) {
   PLF_DYNAMIC(k_c2c, KK_C2C, S, P,   16,  16, 16);
   PLF_DYNAMIC(k_c2c, KK_C2C, S, P,   64,   4, 16);
   PLF_DYNAMIC(k_c2c, KK_C2C, S, P,  256,   1, 16);
+  // 15015 = 3.5.7.11.13 in one LDS pass on a compile-time chain
+  PLF_STATIC(k_c2c, KK_C2C, S, P, 1024,   1, 3, 5, 7, 11, 13);
   PLF_DYNAMIC(k_c2c, KK_C2C, S, P, 1024,   1, 16);
 }
 

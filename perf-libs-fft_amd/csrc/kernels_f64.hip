@@ -57,6 +57,10 @@ void register_kernels_f64(std::vector<KEntry> &r) {
   PLF_STATIC(k_row, KK_ROW, S, P,   64,   8, 8, 8, 8);
   PLF_STATIC(k_row, KK_ROW, S, P,  128,   8, 8, 8, 4, 4);
   PLF_STATIC(k_row, KK_ROW, S, P,  256,   4, 8, 8, 8, 4);
+  // 15015 = 105 x 143: compile-time sides 3.5.7 and 11.13 (16 threads x 16
+  // columns / rows per block)
+  PLF_STATIC(k_col, KK_COL, S, P,   16,  16, 3, 5, 7);
+  PLF_STATIC(k_row, KK_ROW, S, P,   16,  16, 11, 13);
   PLF_DYNAMIC(k_col, KK_COL, S, P,   64,   8, 16);
   PLF_DYNAMIC(k_row, KK_ROW, S, P,   64,   8, 16);
   PLF_DYNAMIC(k_col, KK_COL, S, P,  128,   4, 16);

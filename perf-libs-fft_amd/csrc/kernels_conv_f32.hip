@@ -20,7 +20,9 @@ void register_kernels_conv_f32(std::vector<KEntry> &r) {
     PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 256,   1, 16);
     PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 1024,  1, 16);
     // compile-time chains for the Rader lengths of n = 97 (96) and 1009 (1008)
-    PLF_STATIC(k_rader, KK_RADER, S, P,    8,  32, 4, 8, 3);
+    // 96 on 16 threads x 6 points per member (11 % faster than 8 x 12);
+    // 1008 on 128 threads measured 5-11 % slower than 64
+    PLF_STATIC(k_rader, KK_RADER, S, P,   16,  16, 4, 8, 3);
     PLF_STATIC(k_rader, KK_RADER, S, P,   64,   4, 16, 3, 3, 7);
     PLF_DYNAMIC(k_rader, KK_RADER, S, P,   16,  16, 16);
     PLF_DYNAMIC(k_rader, KK_RADER, S, P,   64,   4, 16);

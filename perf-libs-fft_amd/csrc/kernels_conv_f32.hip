@@ -21,9 +21,11 @@ void register_kernels_conv_f32(std::vector<KEntry> &r) {
     PLF_DYNAMIC(k_bluestein, KK_BLUE, S, P, 1024,  1, 16);
     // compile-time chains for the Rader lengths of n = 97 (96) and 1009 (1008)
     // 96 on 16 threads x 6 points per member (11 % faster than 8 x 12);
-    // 1008 on 128 threads measured 5-11 % slower than 64
+    // 1008 as 16 x 9 x 7 (radix 9 = 3 x 3 in registers: one LDS exchange
+    // fewer than 16 x 3 x 3 x 7, 124 instead of 164 registers in FP32,
+    // 9 % faster; 4 % in FP64); 1008 on 128 threads measured 5-11 % slower
     PLF_STATIC(k_rader, KK_RADER, S, P,   16,  16, 4, 8, 3);
-    PLF_STATIC(k_rader, KK_RADER, S, P,   64,   4, 16, 3, 3, 7);
+    PLF_STATIC(k_rader, KK_RADER, S, P,   64,   4, 16, 9, 7);
     PLF_DYNAMIC(k_rader, KK_RADER, S, P,   16,  16, 16);
     PLF_DYNAMIC(k_rader, KK_RADER, S, P,   64,   4, 16);
     PLF_DYNAMIC(k_rader, KK_RADER, S, P,  256,   1, 16);

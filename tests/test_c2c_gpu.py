@@ -227,7 +227,7 @@ def test_four_step_strided_batch(torch_cuda, orc):
 # fused Bluestein, and chirp-z around the four-step transform; includes the
 # prime configuration's 97, 1009 (Rader), 65537 and the reference's
 # alias-test primes 509, 2053, 8191 (test/plfft_may_alias_test.cpp:40-52).
-PRIMES = [29, 31, 37, 47, 53, 97, 101, 509, 1009, 2053, 4099, 8191, 2 * 1009, 29 * 31]
+PRIMES = [29, 31, 37, 47, 53, 83, 97, 101, 509, 1009, 1399, 2053, 4099, 8191, 2 * 1009, 29 * 31]
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])

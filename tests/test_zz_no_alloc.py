@@ -53,6 +53,22 @@ def test_noalloc_matches_async_two_calls(torch_cuda, orc):
     assert relerr(y2, want) < 2e-6
 
 
+def test_noalloc_rader_three_pass_chunks(torch_cuda, orc):
+    """Rader in three passes (n = 65537): 200 members need 105 MB of
+    workspace with their (x_0, A_0) slots, so the no-alloc call runs chunks
+    of 128 + 72 members, whose slots follow the chunk's workspace."""
+    import plfft_amd as P
+    torch = torch_cuda
+    n, hm = 65537, 200
+    plan, x, ref, y = _case(torch, P, n, hm, P.PLFFT_FORWARD, 14)
+    assert "rader3" in plan.describe()
+    assert plan.workspace_size > P.PLFFT_NOALLOC_SCRATCH_BYTES
+    assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+    k = 3  # a few members against the oracle
+    want = orc.c2c_batched(x[:n * k].astype(np.complex128), n, -1, k, 1, n, 1, n, n * k)
+    assert relerr(y[:n * k], want) < 4e-6
+
+
 def test_noalloc_single_kernel_plan(torch_cuda):
     import plfft_amd as P
     torch = torch_cuda
